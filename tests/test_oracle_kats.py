@@ -1,0 +1,522 @@
+"""Pins the oracle (CPU restatement of render()) to the reference's own
+known-answer tests, golden images and to closed-form / brute-force checks.
+
+Each test names the reference test it transcribes (file:line).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from retrofire_amd import mat as M
+
+F = np.float32
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FAR = 1
+
+
+def v4(x, y, z):
+    return np.array([x, y, z, 1.0], dtype=np.float32)
+
+
+def vtx(x, y, z, a=0.0):
+    """ClipVert<f32>::new(vertex(vec(x,y,z), a)) as (x, y, z, w, attr)."""
+    return [F(x), F(y), F(z), F(1.0), F(a)]
+
+
+def clip(tris, planes):
+    L = O.lib()
+    arr = np.array(tris, dtype=np.float32).reshape(-1)
+    pl = np.array(planes, dtype=np.int32)
+    out = np.zeros(64 * 15, dtype=np.float32)
+    import ctypes as C
+    n = L.oracle_clip_tris_f32(O._p(arr, C.c_float), len(tris), pl.ctypes.data_as(C.POINTER(C.c_int32)),
+                               len(planes), O._p(out, C.c_float), 64)
+    assert n >= 0
+    return out[: n * 15].reshape(n, 3, 5)
+
+
+def clip_poly(plane, verts):
+    import ctypes as C
+    L = O.lib()
+    arr = np.array(verts, dtype=np.float32).reshape(-1)
+    out = np.zeros(5 * 32, dtype=np.float32)
+    n = L.oracle_clip_plane_polygon_f32(plane, O._p(arr, C.c_float), len(verts), O._p(out, C.c_float), 32)
+    return out[: n * 5].reshape(n, 5)
+
+
+# ----------------------------------------------------------------- clip.rs
+
+def test_signed_distance():  # clip.rs:424-430
+    import ctypes as C
+    L = O.lib()
+    sd = lambda p: L.oracle_signed_dist(FAR, O._p(p, C.c_float))
+    assert sd(v4(0, 0, -1)) == -2.0
+    assert sd(v4(1, 0, 0)) == -1.0
+    assert sd(v4(0, 2, 1)) == 0.0
+    assert sd(v4(-3, 0, 2)) == 1.0
+
+
+def test_outcodes():  # clip.rs:432-453
+    import ctypes as C
+    L = O.lib()
+    oc = lambda p: L.oracle_outcode(O._p(p, C.c_float))
+    for p in [(0, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 1)]:
+        assert oc(v4(*p)) == 0
+    assert oc(v4(0, 0, -1.5)) == 0b000001
+    assert oc(v4(2, 0, 0)) == 0b001000
+    assert oc(v4(0, -1.01, 0)) == 0b010000
+    assert oc(v4(-2, 0, 2)) == 0b000110
+
+
+def test_edge_clip():  # clip.rs:455-486
+    e = [vtx(2, 0, -1), vtx(-1, 1, 1)]
+    np.testing.assert_array_equal(clip_poly(FAR, e), np.array(e, np.float32))
+    assert clip_poly(FAR, [vtx(2, 0, 1.5), vtx(-1, 1, 2)]).shape[0] == 0
+    r = clip_poly(FAR, [vtx(2, 0, 0), vtx(-1, 1, 2)])
+    np.testing.assert_array_equal(r[:2], np.array([vtx(2, 0, 0), vtx(0.5, 0.5, 1.0)], np.float32))
+    r = clip_poly(FAR, [vtx(2, 0, 4), vtx(-1, 1, 0)])
+    np.testing.assert_array_equal(r[:2], np.array([vtx(-0.25, 0.75, 1.0), vtx(-1, 1, 0)], np.float32))
+
+
+def _tri(a, b, c):
+    return [vtx(*a), vtx(*b), vtx(*c)]
+
+
+def test_tri_clip_far_plane():  # clip.rs:488-584
+    t = _tri((0, -1, 0), (2, 0, 0.5), (-1, 1.5, 0))
+    np.testing.assert_array_equal(clip([t], [FAR]), np.array([t], np.float32))
+    assert clip([_tri((0, -1, 1.5), (2, 0, 1.5), (-1, 1.5, 2))], [FAR]).shape[0] == 0
+    t = _tri((0, -1, 0), (2, 0, 1), (-1, 1.5, 1))
+    np.testing.assert_array_equal(clip([t], [FAR]), np.array([t], np.float32))
+    out, in1, in2 = (0, 0, 2), (0, 1, 0), (1, 0, 0)
+    np.testing.assert_array_equal(
+        clip([_tri(out, in1, in2)], [FAR]),
+        np.array([_tri((0, 0.5, 1), in1, in2), _tri((0, 0.5, 1), in2, (0.5, 0, 1))], np.float32))
+    on, ins = (1, 0, 1), (0, -1, 0)
+    np.testing.assert_array_equal(clip([_tri(out, on, ins)], [FAR]),
+                                  np.array([_tri(on, ins, (0, -0.5, 1))], np.float32))
+    assert clip([_tri(out, (1, 0, 1), (0, -1, 1))], [FAR]).shape[0] == 0
+
+
+ALL = [0, 1, 2, 3, 4, 5]
+
+
+def test_tri_clip_frustum():  # clip.rs:586-663
+    t = _tri((-1, -1, -1), (1, 1, 0), (0, 1, 1))
+    np.testing.assert_array_equal(clip([t], ALL), np.array([t], np.float32))
+    assert clip([_tri((2, 2, 2), (2, -2, 0), (0, -1, 2))], ALL).shape[0] == 0
+    np.testing.assert_array_equal(
+        clip([_tri((0, 0, 0), (2, 0, 0), (0, 0, 2))], ALL),
+        np.array([_tri((0, 0, 0), (1, 0, 0), (1, 0, 1)), _tri((0, 0, 0), (1, 0, 1), (0, 0, 1))],
+                 np.float32))
+    res = clip([_tri((-1.5, 0, 0), (0, 0, -1.5), (2, 0, 2))], ALL)
+    assert res.shape[0] == 5
+    p = res[..., :4]
+    assert np.all(np.abs(p / p[..., 3:4]) <= F(1.00001))
+
+
+@pytest.mark.slow
+def test_tri_clip_exhaustive_histogram():  # clip.rs:665-719
+    L = O.lib()
+    hist = np.zeros(8, dtype=np.uint64)
+    import ctypes as C
+    oob = L.oracle_clip_histogram(hist.ctypes.data_as(C.POINTER(C.c_uint64)))
+    assert oob == 0
+    assert hist.tolist() == [559754, 536199, 537942, 254406, 58368, 6264, 192, 0]
+    assert int(hist.sum()) == 5 ** 9
+
+
+# --------------------------------------------------------------- raster.rs
+
+def tri_fill(verts):
+    import ctypes as C
+    L = O.lib()
+    v = np.array(verts, dtype=np.float32).reshape(-1)
+    sl = np.zeros(4 * 4096, dtype=np.int64)
+    fr = np.zeros(2 * 65536, dtype=np.float32)
+    nf = C.c_int64(0)
+    n = L.oracle_tri_fill_f32(O._p(v, C.c_float), sl.ctypes.data_as(C.POINTER(C.c_int64)), 4096,
+                              O._p(fr, C.c_float), 65536, C.byref(nf))
+    assert n >= 0
+    return sl[: 4 * n].reshape(n, 4), fr[: 2 * nf.value].reshape(-1, 2)
+
+
+def test_shared_edge_no_gaps_or_overdraw():  # raster.rs:325-368
+    buf = np.zeros((10, 20), dtype=np.int32)
+    vs = [(8.0, 0.0, 0.0, 0.0), (0.0, 6.0, 0.0, 0.0), (14.0, 10.0, 0.0, 0.0), (20.0, 3.0, 0.0, 0.0)]
+    for t in ([vs[0], vs[1], vs[2]], [vs[0], vs[2], vs[3]]):
+        sls, _ = tri_fill(t)
+        for y, x0, x1, _n in sls:
+            for x in range(x0, x1):
+                buf[y, x] += 1
+    expected = """
+00000001110000000000
+00000011111111000000
+00000111111111111100
+00011111111111111111
+00111111111111111110
+01111111111111111100
+00111111111111111000
+00000111111111110000
+00000000011111100000
+00000000000011000000"""
+    got = "".join("\n" + "".join(str(c) for c in row) for row in buf)
+    assert got == expected
+
+
+def test_gradient():  # raster.rs:370-401
+    verts = [(15.0, 2.0, 1.0, 0.0), (2.0, 8.0, 1.0, 1.0), (26.0, 14.0, 1.0, 0.5)]
+    sls, fr = tri_fill(verts)
+    s = "\n"
+    k = 0
+    for y, x0, x1, n in sls:
+        s += " " * max(1, int(x0))
+        for _ in range(n):
+            v = F(F(10.0) * F(fr[k, 1]))
+            s += str(int(v) if v > 0 else 0)  # `as u8` (values are in 0..=10 here)
+            k += 1
+        s += "\n"
+    expected = """
+              0
+            2110
+          3322211
+       55444332221
+     76665544433222
+   88877666554443322
+    98887766655444332
+        88776665544433
+            77666554443
+                66655444
+                    65544
+                        54
+"""
+    assert s == expected
+
+
+def test_scanline_fragments_iter():  # raster.rs:403-436
+    import ctypes as C
+    L = O.lib()
+    w0, w1 = F(2.0), F(4.0)
+    a = np.array([8.0, 42.0, F(1.0) / w0, F(3.0) / w0], dtype=np.float32)
+    b = np.array([16.0, 42.0, F(1.0) / w1, F(5.0) / w1], dtype=np.float32)
+    out = np.zeros(4 * 9, dtype=np.float32)
+    L.oracle_scanline_frags_f32(O._p(a, C.c_float), O._p(b, C.c_float), 9, O._p(out, C.c_float))
+    out = out.reshape(9, 4)
+    zs = [2.0, 2.1333334, 2.2857144, 2.4615386, 2.6666667, 2.909091, 3.2, 3.5555556, 4.0]
+    vars_ = [3.0, 3.1333334, 3.2857144, 3.4615386, 3.6666667, 3.909091, 4.2000003, 4.555556, 5.0]
+    for i in range(9):
+        np.testing.assert_allclose(out[i, :3], [8.0 + i, 42.0, 1.0 / zs[i]], rtol=1e-5)
+        np.testing.assert_allclose(out[i, 3], vars_[i], rtol=1e-5)
+
+
+# ------------------------------------------------------------------ vary.rs
+
+def _vary(start, step, n):
+    import ctypes as C
+    out = np.zeros(max(n, 1), dtype=np.float32)
+    O.lib().oracle_vary_f32(start, step, n, O._p(out, C.c_float))
+    return out[:n]
+
+
+def _vary_to(a, b, n):
+    import ctypes as C
+    out = np.zeros(max(n, 1), dtype=np.float32)
+    O.lib().oracle_vary_to_f32(a, b, n, O._p(out, C.c_float))
+    return out[:n]
+
+
+def test_vary():  # vary.rs:38-45, 59-64, 163-191
+    np.testing.assert_array_equal(_vary(0.0, 0.2, 5), np.array([0.0, 0.2, 0.4, 0.6, 0.8], np.float32))
+    np.testing.assert_allclose(_vary(-6.0, 1.2, 10),
+                               [-6.0, -4.8, -3.6, -2.4, -1.2, 0.0, 1.2, 2.4, 3.6, 4.8], atol=1e-5)
+    np.testing.assert_array_equal(_vary_to(2.0, 8.0, 3), [2.0, 5.0, 8.0])
+    assert _vary_to(1.0, 2.0, 0).size == 0
+    np.testing.assert_array_equal(_vary_to(1.0, 2.0, 1), [1.0])
+    np.testing.assert_array_equal(_vary_to(1.0, 2.0, 2), [1.0, 2.0])
+
+
+# ------------------------------------------------------------------- tex.rs
+
+def _sample(kind, u, v):
+    import ctypes as C
+    tex = np.array([[[0xFF, 0, 0, 0xFF], [0, 0xFF, 0, 0xFF]],
+                    [[0, 0, 0xFF, 0xFF], [0xFF, 0xFF, 0, 0xFF]]], dtype=np.uint8)
+    out = np.zeros(4, dtype=np.uint8)
+    assert O.lib().oracle_sample(kind, tex.ctypes.data_as(C.POINTER(C.c_uint8)), 2, 2, 2, u, v,
+                                 out.ctypes.data_as(C.POINTER(C.c_uint8))) == 0
+    return tuple(out[:3].tolist())
+
+
+R, G, B, Y = (0xFF, 0, 0), (0, 0xFF, 0), (0, 0, 0xFF), (0xFF, 0xFF, 0)
+
+
+def test_sampler_repeat_pot():  # tex.rs:380-393
+    s = lambda u, v: _sample(3, u, v)
+    assert s(-0.1, 0.0) == G and s(0.0, -0.1) == B
+    assert s(1.0, 0.0) == R and s(0.0, 1.0) == R
+    assert s(4.8, 0.2) == G and s(0.2, 4.8) == B
+
+
+def test_sampler_clamp():  # tex.rs:395-406
+    s = lambda u, v: _sample(2, u, v)
+    assert s(-1.0, 0.0) == R and s(0.0, -1.0) == R
+    assert s(1.5, 0.0) == G and s(0.0, 1.5) == B
+    assert s(1.5, 1.5) == Y
+
+
+# ------------------------------------------------------- golden images
+
+def read_ppm(path):
+    data = open(path, "rb").read()
+    parts, i = [], 0
+    while len(parts) < 4:
+        while data[i:i + 1].isspace():
+            i += 1
+        j = i
+        while not data[j:j + 1].isspace():
+            j += 1
+        parts.append(data[i:j])
+        i = j
+    i += 1
+    assert parts[0] == b"P6" and parts[3] == b"255"
+    w, h = int(parts[1]), int(parts[2])
+    return np.frombuffer(data[i:i + w * h * 3], dtype=np.uint8).reshape(h, w, 3)
+
+
+def textured_quad_inputs():
+    verts = np.array([[-1, -1, 0], [1, -1, 0], [-1, 1, 0], [1, 1, 0]], np.float32)
+    uvs = np.array([[0, 0], [0, 1], [1, 0], [1, 1]], np.float32)
+    faces = np.array([[0, 1, 2], [3, 2, 1]], np.uint32)
+    tex = np.zeros((8, 8, 4), np.uint8)
+    for y in range(8):
+        for x in range(8):
+            xor = (x ^ y) & 1
+            tex[y, x] = (0x7F * xor, 0, 0xFF * (1 - xor), 0)
+    project = M.perspective(1.0, 1.0, 0.1, 1000.0)
+    vp = M.viewport(0, 0, 256, 256)
+    mvp = M.then(M.translate3(0.0, 0.0, 1.0), project)
+    return verts, uvs, faces, tex, mvp, vp
+
+
+def test_textured_quad_golden():  # core/tests/rendering.rs:18-60
+    verts, uvs, faces, tex, mvp, vp = textured_quad_inputs()
+    fb = np.zeros((256, 256, 3), np.uint8)  # Buf2<Color3>: colour-only target
+    err, st = O.render(2, faces, verts, uvs, mvp, vp, fb, fmt=1, tex=tex)
+    assert err == 0
+    assert tuple(fb[0, 0]) == (0, 0, 0xFF)
+    assert tuple(fb[255, 0]) == (0x7F, 0, 0)
+    assert tuple(fb[0, 255]) == (0x7F, 0, 0)
+    gold = read_ppm(os.path.join(GOLDEN, "textured_quad.ppm"))
+    np.testing.assert_array_equal(fb, gold)
+    assert int(st[5]) == 256 * 256  # every pixel covered once
+
+
+def hello_tri_inputs():
+    verts = np.array([[-1, 1, 0], [1, 1, 0], [0, -1, 0]], np.float32)
+    cols = np.array([[1.0, 0.0, 0.0], [0.0, 0.8, 0.0], [0.4, 0.4, 1.0]], np.float32)
+    faces = np.array([[0, 1, 2]], np.uint32)
+    w, h = 640, 480
+    modelview = M.translate3(0.0, 0.0, 2.0)
+    project = M.perspective(1.0, F(w) / F(h), 0.1, 1000.0)
+    vp = M.viewport(0, h, w, 0)
+    return verts, cols, faces, M.then(modelview, project), vp
+
+
+def test_hello_tri_center_pixel():  # core/examples/hello_tri.rs:47-53 (no `fp`)
+    verts, cols, faces, mvp, vp = hello_tri_inputs()
+    fb = np.zeros((480, 640, 4), np.uint8)  # Buf2<Color4>
+    err, st = O.render(1, faces, verts, cols, mvp, vp, fb, fmt=0)
+    assert err == 0
+    assert tuple(fb[240, 320]) == (114, 102, 128, 255)
+
+
+def test_hello_tri_fp_golden_coverage():  # core/triangle.ppm, hello_tri.rs `fp` variant
+    import ctypes as C
+    verts, cols, faces, mvp, vp = hello_tri_inputs()
+    fb = np.zeros((480, 640, 4), np.uint8)
+    st = np.zeros(8, np.uint64)
+    L = O.lib()
+    err = L.oracle_render_hello_fp(O._p(faces.reshape(-1), C.c_uint32), 1, O._p(verts.reshape(-1), C.c_float),
+                                   O._p(cols.reshape(-1), C.c_float), 3, O._p(mvp.reshape(-1), C.c_float),
+                                   O._p(vp.reshape(-1), C.c_float), fb.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                   640, 480, O._p(st, C.c_uint64))
+    assert err == 0
+    gold = read_ppm(os.path.join(GOLDEN, "triangle.ppm"))
+    cov_gold = gold.any(axis=2)
+    cov = fb[..., 3] == 255
+    assert int(cov_gold.sum()) == 51200 == 320 * 320 // 2
+    np.testing.assert_array_equal(cov, cov_gold)
+    assert tuple(fb[240, 320, :3]) == (151, 128, 187)
+    diff = np.abs(fb[..., :3].astype(int) - gold.astype(int))
+    assert diff.max() <= 1  # colours go through libm powf: +-1 LSB
+
+
+# ------------------------------------------- closed-form and brute-force pins
+
+def test_axis_aligned_rect_exact_coverage():
+    # Two triangles forming the rectangle [xa, xb] x [ya, yb] in pixels (viewport
+    # y-down, identity mvp). Under round_up_to_half (raster.rs:267-307) a pixel
+    # is covered iff its centre lies in (xa, xb] x (ya, yb]: a centre exactly on
+    # the left/top edge is outside, on the right/bottom edge inside.
+    w, h = 64, 48
+    vp = M.viewport(0, 0, w, h)
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        xa, xb = sorted(rng.integers(0, 2 * w, 2) / 2.0)
+        ya, yb = sorted(rng.integers(0, 2 * h, 2) / 2.0)
+        if xb - xa < 1 or yb - ya < 1:
+            continue
+        ndc = lambda x, y: (F(x) / F(w / 2) - F(1), F(y) / F(h / 2) - F(1), F(0))
+        verts = np.array([ndc(xa, ya), ndc(xb, ya), ndc(xa, yb), ndc(xb, yb)], np.float32)
+        faces = np.array([[0, 2, 1], [1, 2, 3]], np.uint32)
+        fb = np.zeros((h, w, 4), np.uint8)
+        err, st = O.render(0, faces, verts, None, M.identity(), vp, fb, ctx=(0, 0, 1, 1))
+        assert err == 0
+        cov = fb[..., 3] == 255
+        cx = np.arange(w) + 0.5
+        cy = np.arange(h) + 0.5
+        exp = ((cx[None, :] > xa) & (cx[None, :] <= xb)) & ((cy[:, None] > ya) & (cy[:, None] <= yb))
+        # The shared diagonal runs from (xb, ya) to (xa, yb); a pixel centre that
+        # coincides with either endpoint is decided by stepped float edge values
+        # and may go either way.
+        ok = np.ones_like(exp)
+        for vx, vy in ((xb, ya), (xa, yb)):
+            if vx - 0.5 == int(vx - 0.5) and vy - 0.5 == int(vy - 0.5) and 0 <= vx < w and 0 <= vy < h:
+                ok[int(vy), int(vx)] = False
+        np.testing.assert_array_equal(cov[ok], exp[ok])
+        assert int(st[5]) == int(cov.sum())
+
+
+def _edge(ax, ay, bx, by, px, py):
+    return (bx - ax) * (py - ay) - (by - ay) * (px - ax)
+
+
+def test_brute_force_coverage_random_triangles():
+    """Coverage equals the point-in-triangle test at pixel centres, except for
+    centres within 1e-3 px of an edge (where the reference's stepped float edges
+    may legitimately decide either way)."""
+    w, h = 48, 40
+    vp = M.viewport(0, 0, w, h)
+    rng = np.random.default_rng(7)
+    checked = 0
+    for _ in range(200):
+        pts = rng.uniform([1, 1], [w - 1, h - 1], size=(3, 2)).astype(np.float32)
+        ndc = np.array([[F(x) / F(w / 2) - F(1), F(y) / F(h / 2) - F(1), 0] for x, y in pts], np.float32)
+        scr = np.array([M.apply_pt(vp, p)[:2] for p in ndc], np.float64)
+        fb = np.zeros((h, w, 4), np.uint8)
+        err, st = O.render(0, np.array([[0, 1, 2]], np.uint32), ndc, None, M.identity(), vp, fb,
+                           ctx=(0, 0, 1, 1))
+        assert err == 0
+        cov = fb[..., 3] == 255
+        (ax, ay), (bx, by), (cx, cy) = scr
+        area = _edge(ax, ay, bx, by, cx, cy)
+        if abs(area) < 1.0:
+            continue
+        px, py = np.meshgrid(np.arange(w) + 0.5, np.arange(h) + 0.5)
+        e0 = _edge(ax, ay, bx, by, px, py) / area
+        e1 = _edge(bx, by, cx, cy, px, py) / area
+        e2 = _edge(cx, cy, ax, ay, px, py) / area
+        inside = (e0 > 0) & (e1 > 0) & (e2 > 0)
+        lens = [np.hypot(bx - ax, by - ay), np.hypot(cx - bx, cy - by), np.hypot(ax - cx, ay - cy)]
+        dist = np.minimum.reduce([np.abs(e0 * area) / lens[0], np.abs(e1 * area) / lens[1],
+                                  np.abs(e2 * area) / lens[2]])
+        clear = dist > 1e-3
+        np.testing.assert_array_equal(cov[clear], inside[clear])
+        assert int(st[5]) == int(cov.sum())
+        checked += 1
+    assert checked > 150
+
+
+def test_perspective_correct_uv_matches_ray_plane():
+    """At every covered pixel, the sampled texel of a textured quad in 3D equals
+    the texel at the ray / plane intersection (textbook projective mapping),
+    except within 0.02 texel of a texel border."""
+    w, h = 96, 64
+    tex_n = 16
+    tex = np.zeros((tex_n, tex_n, 4), np.uint8)
+    tex[..., 0] = np.arange(tex_n)[None, :]
+    tex[..., 1] = np.arange(tex_n)[:, None]
+    tex[..., 3] = 255
+    quad = np.array([[-1, -0.6, 0], [1, -0.6, 0], [-1, 0.6, 0], [1, 0.6, 0]], np.float32)
+    uvs = np.array([[0, 0], [1, 0], [0, 1], [1, 1]], np.float32)
+    faces = np.array([[0, 1, 2], [3, 2, 1]], np.uint32)
+    model = M.then(M.rotate_y(0.11), M.translate3(0.1, 0.05, 2.5))
+    model = np.array(model)
+    model[1, 2] = F(0.4)  # shear the plane in depth so w varies across rows and columns
+    proj = M.perspective(1.2, F(w) / F(h), 0.1, 100.0)
+    mvp = M.then(model, proj)
+    vp = M.viewport(0, h, w, 0)
+    fb = np.zeros((h, w, 4), np.uint8)
+    depth = np.zeros((h, w), np.float32)
+    err, st = O.render(2, faces, quad, uvs, mvp, vp, fb, depth=depth, tex=tex, ctx=(1, 0, 1, 1))
+    assert err == 0
+    cov = fb[..., 3] == 255
+    assert cov.sum() > 1000
+    # invert: pixel centre -> NDC -> ray in model space -> plane z=0 -> uv
+    full = np.array(M.then(mvp, vp), dtype=np.float64)  # not used for mapping
+    m = np.array(mvp, dtype=np.float64)
+    v = np.array(vp, dtype=np.float64)
+    mismatches = 0
+    total = 0
+    for py in range(h):
+        for px in range(w):
+            if not cov[py, px]:
+                continue
+            sx, sy = px + 0.5, py + 0.5
+            nx = (sx - v[0, 3]) / v[0, 0]
+            ny = (sy - v[1, 3]) / v[1, 1]
+            # solve m @ [x, y, 0, 1] = wc * [nx, ny, *, 1] for x, y, wc
+            A = np.array([[m[0, 0], m[0, 1], -nx], [m[1, 0], m[1, 1], -ny], [m[3, 0], m[3, 1], -1.0]])
+            rhs = -np.array([m[0, 3], m[1, 3], m[3, 3]])
+            x, y, _ = np.linalg.solve(A, rhs)
+            u = (x + 1) / 2 * tex_n
+            vv = (y + 0.6) / 1.2 * tex_n
+            if min(abs(u - round(u)), abs(vv - round(vv))) < 0.02:
+                continue
+            total += 1
+            if (fb[py, px, 0], fb[py, px, 1]) != (int(np.floor(u)) % tex_n, int(np.floor(vv)) % tex_n):
+                mismatches += 1
+    assert total > 500
+    assert mismatches == 0
+
+
+def test_depth_test_first_drawn_wins_ties():
+    """ctx.rs:86-89: a fragment passes iff curr < new; on equal z the first
+    drawn triangle keeps the pixel, and frags.o counts only passing writes."""
+    w, h = 32, 32
+    vp = M.viewport(0, h, w, 0)
+    verts = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32)
+    mvp = M.then(M.translate3(0, 0, 3), M.perspective(1.0, 1.0, 0.1, 10.0))
+    fb = np.zeros((h, w, 4), np.uint8)
+    depth = np.zeros((h, w), np.float32)
+    st = np.zeros(8, np.uint64)
+    O.render(0, np.array([[0, 1, 2]]), verts, None, mvp, vp, fb, depth=depth, flat=(10, 20, 30, 255),
+             ctx=(1, 0, 1, 1), stats=st)
+    first_o = int(st[6])
+    assert first_o > 0 and int(st[5]) == first_o
+    O.render(0, np.array([[0, 1, 2]]), verts, None, mvp, vp, fb, depth=depth, flat=(90, 80, 70, 255),
+             ctx=(1, 0, 1, 1), stats=st)
+    cov = fb[..., 3] == 255
+    assert np.all(fb[cov][:, :3] == (10, 20, 30))
+    assert int(st[6]) == first_o and int(st[5]) == 2 * first_o
+    # Equal test with depth writes re-draws exactly the same fragments
+    O.render(0, np.array([[0, 1, 2]]), verts, None, mvp, vp, fb, depth=depth, flat=(1, 2, 3, 255),
+             ctx=(2, 0, 1, 1), stats=st)
+    assert np.all(fb[cov][:, :3] == (1, 2, 3))
+
+
+def test_index_out_of_range_is_an_error():  # prim.rs:17-19 panics on a bad index
+    fb = np.zeros((8, 8, 4), np.uint8)
+    err, _ = O.render(0, np.array([[0, 1, 3]]), np.zeros((3, 3), np.float32), None, M.identity(),
+                      M.viewport(0, 0, 8, 8), fb)
+    assert err == -1
+
+
+def test_span_outside_target_is_an_error():  # target.rs:173-174 panics out of bounds
+    fb = np.zeros((8, 8, 4), np.uint8)
+    verts = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32)
+    err, _ = O.render(0, np.array([[0, 1, 2]]), verts, None, M.identity(), M.viewport(0, 0, 16, 16),
+                      fb, ctx=(0, 0, 1, 1))
+    assert err == -2
