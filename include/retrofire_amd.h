@@ -1,0 +1,240 @@
+/*
+ * retrofire_amd: the triangle rendering path of retrofire
+ * (`retrofire_core::render::render`, core/src/render.rs:134-207) on an
+ * MI355X (gfx950), behind a C ABI.
+ *
+ * The reference has no C ABI: its interface is the generic Rust function
+ *
+ *   pub fn render<Prim, Vtx, Var, Uni, Shd>(prims, verts, shader: &Shd,
+ *       uniform: Uni, to_screen: Mat4<Ndc, Screen>, target: &mut impl Target,
+ *       ctx: &Context)                                   (render.rs:134-146)
+ *
+ * monomorphised over the shader and target types. The entry points below are
+ * what a Rust FFI crate binds to replace that call when the shader is one of
+ * the built-in kinds and the target one of the supported buffer layouts; any
+ * other shader stays on the reference's own CPU path.
+ *
+ * Conventions
+ *   - Every function returns an rf_status (0 = success, negative = error).
+ *   - Matrices are row-major float[16], applied as row . column
+ *     (mat.rs:929-970): `mvp` is the uniform of the built-in vertex shader
+ *     `vertex(mvp.apply(&v.pos), v.attrib)`, `viewport` is `to_screen`.
+ *   - Index triples are uint32 (the reference's Tri<usize>).
+ *   - One rf_device per host thread, like the reference's !Sync Context
+ *     (ctx.rs:64). A serving handle owns its own worker thread.
+ */
+#ifndef RETROFIRE_AMD_H
+#define RETROFIRE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rf_status;
+#define RF_OK 0
+#define RF_E_INDEX (-1)       /* vertex index out of range (prim.rs:17-19 panics)   */
+#define RF_E_BOUNDS (-2)      /* span outside the target (target.rs:173-174 panics) */
+#define RF_E_ARG (-3)         /* invalid argument (e.g. non-power-of-two texture for
+                                 SamplerRepeatPot, tex.rs:228-233 panics)            */
+#define RF_E_DEVICE (-4)      /* HIP runtime error or out of device memory         */
+#define RF_E_UNSUPPORTED (-5) /* not a built-in shader kind / target layout, or
+                                 depth_sort set: the caller keeps the CPU path      */
+#define RF_E_VIEWPORT (-6)    /* to_screen maps the NDC cube outside the target   */
+#define RF_E_NONFINITE (-7)   /* a screen-space vertex is NaN or infinite          */
+#define RF_E_INTERNAL (-8)    /* internal capacity or binning check failed         */
+#define RF_E_BUDGET (-9)      /* request larger than the serving memory budget     */
+
+/* Built-in shader kinds (shader.rs:57-126 closure and Shader forms). */
+typedef enum rf_shader_kind {
+    RF_SHADER_FLAT = 0,            /* |_| color: constant Color4, varying ()            */
+    RF_SHADER_GOURAUD = 1,         /* Color3f varying, frag.var.to_color4()             */
+    RF_SHADER_TEX_CLAMP = 2,       /* TexCoord varying, SamplerClamp.sample (tex.rs:272) */
+    RF_SHADER_TEX_REPEAT_POT = 3   /* TexCoord varying, SamplerRepeatPot (tex.rs:218)    */
+} rf_shader_kind;
+
+/* Colour buffer element layouts (target.rs:35-136, pixfmt.rs:76-97). */
+typedef enum rf_pixfmt {
+    RF_PIX_RGBA8 = 0,      /* [u8; 4] r,g,b,a: Buf2<Color4> or Colorbuf<_, Rgba8888> */
+    RF_PIX_RGB8 = 1,       /* [u8; 3] r,g,b:   Buf2<Color3>                          */
+    RF_PIX_U32_RGBA = 2,   /* u32 0xRRGGBBAA:  Colorbuf<Buf2<u32>, Rgba8888>          */
+    RF_PIX_U32_XRGB = 3    /* u32 0x00RRGGBB:  Colorbuf<Buf2<u32>, Xrgb8888>          */
+} rf_pixfmt;
+
+/* Context (ctx.rs:11-65). depth_test holds the reference's
+ * Option<Ordering>: 0 = None, 1 = Less, 2 = Equal, 3 = Greater. face_cull:
+ * 0 = None, 1 = Back, 2 = Front. depth_sort must be 0 (None); other values
+ * return RF_E_UNSUPPORTED. clear_color / clear_depth fuse the frontend's
+ * Frame::clear (front/src/lib.rs:103-120) into the call: the colour buffer is
+ * first filled with color_clear, the depth buffer with 1 / depth_clear. */
+typedef struct rf_ctx {
+    int32_t depth_test;
+    int32_t face_cull;
+    int32_t color_write;
+    int32_t depth_write;
+    int32_t depth_sort;
+    int32_t clear_color;
+    uint8_t color_clear[4]; /* Color4 r, g, b, a */
+    int32_t clear_depth;
+    float depth_clear;      /* default +inf: the buffer is filled with 1/inf = 0 */
+} rf_ctx;
+
+/* Stats (stats.rs:16-40). `seconds` is the wall time of the call. */
+typedef struct rf_stats {
+    uint64_t calls;
+    uint64_t prims_i, prims_o;
+    uint64_t verts_i, verts_o;
+    uint64_t frags_i, frags_o;
+    double seconds;
+} rf_stats;
+
+/* A render target: a colour buffer and an optional f32 depth buffer holding
+ * 1/w (target.rs:35-44). Pointers are host memory for rf_render and device
+ * memory (from rf_framebuf_create) for the batched and serving paths. */
+typedef struct rf_target {
+    void* color;
+    uint32_t width, height;
+    uint32_t color_stride;   /* bytes per row */
+    uint32_t fmt;            /* rf_pixfmt */
+    float* depth;            /* NULL: colour-only target, no depth test */
+    uint32_t depth_stride;   /* floats per row */
+} rf_target;
+
+/* One render() call with host-resident data: vertices are position (x,y,z)
+ * plus an attribute of 0 (flat), 3 (Gouraud, Color3f) or 2 (textured,
+ * TexCoord) floats. */
+typedef struct rf_draw {
+    uint32_t kind;                  /* rf_shader_kind */
+    const uint32_t* tris;           /* n_tris x 3 indices */
+    uint64_t n_tris;
+    const float* pos;               /* n_verts x 3 */
+    const float* attr;              /* n_verts x attribute floats */
+    uint64_t n_verts;
+    float mvp[16];
+    float viewport[16];
+    uint8_t flat_color[4];          /* RF_SHADER_FLAT: Color4 r, g, b, a */
+    const uint8_t* tex;             /* textured: Color4 texels, row-major */
+    uint32_t tex_w, tex_h, tex_stride; /* stride in texels */
+} rf_draw;
+
+typedef struct rf_device rf_device;
+typedef struct rf_mesh rf_mesh;
+typedef struct rf_texture rf_texture;
+typedef struct rf_framebuf rf_framebuf;
+typedef struct rf_serve rf_serve;
+
+/* ---- device ---------------------------------------------------------- */
+rf_status rf_device_open(int ordinal, rf_device** out);
+rf_status rf_device_close(rf_device* dev);
+const char* rf_status_str(rf_status s);
+/* Bytes of device memory the library holds for scratch and residency. */
+uint64_t rf_device_bytes(const rf_device* dev);
+
+/* ---- plain path: the drop-in for render() ------------------------------
+ * Synchronous, like the reference call: uploads the draw's data and the
+ * target, runs the pipeline, reads colour (and depth) back before returning.
+ * Replaces render() (core/src/render.rs:134-207) with the built-in shader of
+ * `draw->kind`, `uniform` = draw->mvp and `to_screen` = draw->viewport.
+ * Stats are added to *stats (may be NULL). */
+rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, const rf_ctx* ctx,
+                    rf_stats* stats);
+
+/* ---- residency ---------------------------------------------------------
+ * Device-resident meshes and textures (geom/mesh.rs:24-30, tex.rs:33-37).
+ * n_attr is the attribute float count of the kinds the mesh is drawn with. */
+rf_status rf_mesh_create(rf_device* dev, const float* pos, const float* attr, uint32_t n_attr,
+                         uint64_t n_verts, const uint32_t* tris, uint64_t n_tris, rf_mesh** out);
+rf_status rf_mesh_free(rf_device* dev, rf_mesh* mesh);
+rf_status rf_texture_create(rf_device* dev, const uint8_t* rgba, uint32_t w, uint32_t h,
+                            uint32_t stride_texels, rf_texture** out);
+rf_status rf_texture_free(rf_device* dev, rf_texture* tex);
+/* A device-resident target; rf_framebuf_target fills an rf_target with its
+ * device pointers. */
+rf_status rf_framebuf_create(rf_device* dev, uint32_t w, uint32_t h, uint32_t fmt, int with_depth,
+                             rf_framebuf** out);
+rf_status rf_framebuf_free(rf_device* dev, rf_framebuf* fb);
+rf_status rf_framebuf_target(const rf_framebuf* fb, rf_target* out);
+rf_status rf_framebuf_read(rf_device* dev, const rf_framebuf* fb, void* color, float* depth);
+
+/* ---- batched path -------------------------------------------------------
+ * Many frames of uneven resolution, each an ordered list of draws over
+ * resident meshes, go through one pipeline launch. */
+typedef struct rf_mesh_draw {
+    const rf_mesh* mesh;
+    uint32_t kind;
+    float mvp[16];
+    float viewport[16];
+    uint8_t flat_color[4];
+    const rf_texture* tex;
+    uint64_t first_tri;      /* draw triangles [first_tri, first_tri + n_tris) */
+    uint64_t n_tris;         /* 0: to the end of the mesh */
+} rf_mesh_draw;
+
+typedef struct rf_frame {
+    rf_framebuf* target;
+    rf_ctx ctx;
+    const rf_mesh_draw* draws;
+    uint32_t n_draws;
+} rf_frame;
+
+typedef struct rf_batch rf_batch;
+/* Plans a batch: sizes scratch from one eager run, picks the tile width
+ * (tile_w = 0: autotune; 32 or 64 to force) and captures the launch sequence
+ * into a HIP graph. The frames' descriptors are copied; the meshes, textures
+ * and framebuffers must stay alive while the batch exists. */
+rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_frames,
+                          uint32_t tile_w, rf_batch** out);
+/* Replays the batch; stats (n_frames entries, may be NULL) are per frame.
+ * `sync` = 0 returns without waiting (rf_batch_wait then waits). */
+rf_status rf_batch_run(rf_batch* b, int sync);
+rf_status rf_batch_wait(rf_batch* b, rf_stats* stats);
+rf_status rf_batch_free(rf_batch* b);
+uint32_t rf_batch_tile_width(const rf_batch* b);
+/* Kernel-level timing of the last replay (HIP events on the batch stream):
+ * total milliseconds and the rasterizer's share. */
+rf_status rf_batch_last_ms(const rf_batch* b, float* total_ms, float* raster_ms);
+/* Runs the pipeline `iters` times outside the graph with an event after each
+ * stage on the batch stream; ms[0..5] receive the mean milliseconds of:
+ * vertex (with counter reset), setup, scan, bin, bin sort, raster. */
+rf_status rf_batch_profile(rf_batch* b, int iters, float* ms);
+
+/* ---- serving ------------------------------------------------------------
+ * A request loop that keeps meshes and textures resident, admits frames by a
+ * device-memory budget and batches whatever is admitted while earlier work is
+ * still on the GPU (continuous batching over `n_streams` streams). */
+typedef struct rf_request {
+    uint64_t id;
+    const rf_mesh_draw* draws;   /* copied at submit */
+    uint32_t n_draws;
+    uint32_t width, height, fmt;
+    int32_t with_depth;
+    rf_ctx ctx;
+} rf_request;
+
+typedef struct rf_result {
+    uint64_t id;
+    rf_status status;
+    uint32_t width, height, fmt;
+    double submit_s, start_s, done_s;   /* monotonic clock seconds */
+    rf_stats stats;
+} rf_result;
+
+rf_status rf_serve_open(rf_device* dev, uint64_t budget_bytes, uint32_t max_batch,
+                        uint32_t n_streams, rf_serve** out);
+/* Non-blocking; the result is delivered through rf_serve_poll. */
+rf_status rf_serve_submit(rf_serve* s, const rf_request* req);
+/* Copies up to `max` finished results; with `color` non-NULL, result i's
+ * pixels are copied to color[i] (width * height * bpp bytes). Blocks up to
+ * timeout_ms for at least one result (0 = do not block). */
+rf_status rf_serve_poll(rf_serve* s, rf_result* out, void** color, uint32_t max,
+                        uint32_t* n_out, uint32_t timeout_ms);
+rf_status rf_serve_close(rf_serve* s);
+double rf_now_s(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,59 @@
+"""Builds the HIP library (retrofire_amd/librfamd.so) for gfx950 with hipcc.
+
+Float semantics: -ffp-contract=off keeps every a*b+c as two rounded
+operations (the reference never fuses), and the default correctly rounded
+single-precision division and square root are kept (no fast-math).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "librfamd.so")
+SOURCES = ["rf_kernels.hip", "rf_api.cpp", "rf_batch.cpp", "rf_serve.cpp"]
+HEADERS = ["rf_types.hpp", "rf_math.hpp", "rf_internal.hpp"]
+ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = [
+    "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    "-I", os.path.join(ROOT, "include"),
+]
+
+
+def _objs():
+    return [os.path.join(CSRC, "_build", os.path.splitext(s)[0] + ".o") for s in SOURCES]
+
+
+def build(verbose=False, jobs=4):
+    os.makedirs(os.path.join(CSRC, "_build"), exist_ok=True)
+    procs = []
+    for src, obj in zip(SOURCES, _objs()):
+        cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+    failed = False
+    for src, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            failed = True
+            sys.stderr.write(f"--- {src}\n{out.decode(errors='replace')}\n")
+        elif verbose and out:
+            sys.stderr.write(out.decode(errors="replace"))
+    if failed:
+        raise RuntimeError("hipcc failed")
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + _objs() + ["-lpthread"]
+    subprocess.check_call(cmd)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,81 @@
+// Internal host-side structures of the runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <vector>
+
+#include "../../include/retrofire_amd.h"
+#include "rf_math.hpp"
+#include "rf_types.hpp"
+
+namespace rf {
+
+constexpr uint32_t WAVE_SIZE = 64;
+constexpr uint32_t TILE_ROWS = 64;          // rasterizer tile height (one lane per row)
+constexpr uint32_t RASTER_WG_PER_CU = 4;    // persistent rasterizer workgroups per CU
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes, uint64_t* counter);
+    void release(uint64_t* counter);
+};
+
+// Per-pipeline scratch buffers.
+struct Scratch {
+    DevBuf cv, ca, sv, hdr, scr, extra, rect, run_count, run_off, run_cursor, runs, misc, stats;
+    hipError_t prepare(BatchDesc& B, uint64_t* counter);
+    void release(uint64_t* counter);
+};
+
+struct Device {
+    int ordinal = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    uint64_t bytes = 0;  // device memory held
+    Scratch plain;
+    DevBuf up_pos, up_attr, up_tris, up_tex, up_color, up_depth, up_desc;
+    uint32_t plain_tile_w = 64;
+    uint32_t plain_extra_cap = 0, plain_runs_cap = 0;
+};
+
+struct Mesh {
+    DevBuf pos, attr, tris;
+    uint64_t n_verts = 0, n_tris = 0;
+    uint32_t n_attr = 0;
+};
+
+struct Tex {
+    DevBuf data;
+    uint32_t w = 0, h = 0, stride = 0;
+};
+
+struct Framebuf {
+    DevBuf color, depth;
+    uint32_t w = 0, h = 0, fmt = 0, stride = 0;
+    bool has_depth = false;
+};
+
+double now_s();
+uint32_t bytes_per_pixel(uint32_t fmt);
+void fill_frame_ctx(FrameDesc& f, const rf_ctx& c);
+rf_status check_ctx(const rf_ctx* c);
+rf_status check_tex(uint32_t kind, const void* tex, uint32_t w, uint32_t h, uint32_t stride);
+rf_status map_err(uint32_t e);
+void set_grid(FrameDesc& f, uint32_t tile_w, uint32_t& tile_base);
+uint32_t raster_grid(const Device& d, uint32_t n_tiles);
+rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const std::vector<FrameStats*>&,
+                    std::vector<FrameStats>& stats_out, uint32_t* err_out,
+                    const std::function<hipError_t()>& restore);
+
+// Enqueues the whole pipeline for batch `B` on stream `s` (rf_kernels.hip).
+hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid);
+// Same, with an event recorded after each stage (for per-kernel timing);
+// `ev` holds 8 events.
+hipError_t launch_pipeline_timed(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev);
+
+}  // namespace rf

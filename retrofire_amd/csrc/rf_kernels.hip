@@ -1,0 +1,994 @@
+// HIP kernels of the render path for gfx950 (CDNA4, wave64).
+//
+//   k_vertex   one thread per vertex slot: the built-in vertex shader
+//              (ProjMat3::apply, mat.rs:965-969), the outcode (clip.rs:303-309)
+//              and the screen-space vertex (prim.rs:62-88) used when the
+//              triangle needs no clipping.
+//   k_setup    one wave per group of 64 consecutive triangles: primitive
+//              assembly (prim.rs:17-19), frustum clipping (clip.rs:350-400),
+//              to_screen, face culling (render.rs:185-196), the tile rectangle
+//              of each triangle, and the per-tile count of bin runs.
+//   k_scan     exclusive scan of the run counts.
+//   k_bin      one wave per group again: writes one run (group, 64-bit mask)
+//              per distinct tile the group touches.
+//   k_binsort  one workgroup per tile: sorts the tile's runs by group, which
+//              restores primitive submission order inside the tile.
+//   k_raster   one wave per tile (persistent): the tile's colour and depth
+//              live in LDS; each lane owns one tile row and walks that row's
+//              spans in submission order (tri_fill/scan raster.rs:185-302,
+//              ScanlineIter::next raster.rs:84-113, rasterize_fb
+//              target.rs:163-198), then the tile is written to HBM once.
+//
+// Exactness: the serial recurrences of the reference (edge stepping per row,
+// varying stepping per pixel) are reproduced as serial sums; no closed forms.
+#include <hip/hip_runtime.h>
+
+#include "rf_math.hpp"
+#include "rf_types.hpp"
+
+namespace rf {
+
+constexpr int WAVE = 64;
+constexpr int TILE_H = 64;  // one lane per tile row
+constexpr int BIN_MARGIN = 2;  // extra pixels around a triangle's bounding box
+
+// ---------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, WAVE);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+// Exclusive prefix sum over the wave.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) {
+    const uint32_t lane = lane_id();
+    uint32_t x = v;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, WAVE);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    *total = __shfl(x, WAVE - 1, WAVE);
+    return x - v;
+}
+
+__device__ __forceinline__ uint32_t find_draw_tri(const DrawDesc* d, uint32_t n, uint64_t g) {
+    uint32_t lo = 0, hi = n;  // last draw with tri_base <= g
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (d[mid].tri_base <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t find_draw_vert(const DrawDesc* d, uint32_t n, uint64_t s) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (d[mid].vert_base <= s) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// k_vertex
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_vertex(BatchDesc B) {
+    const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= B.n_verts) return;
+    const uint32_t di = find_draw_vert(B.draws, B.n_draws, slot);
+    const DrawDesc* D = &B.draws[di];
+    const uint64_t i = slot - D->vert_base;
+    const float px = D->pos[3 * i + 0], py = D->pos[3 * i + 1], pz = D->pos[3 * i + 2];
+    // vertex(mvp.apply(&v.pos), v.attrib): ProjMat3::apply, to_hom w = 1
+    const float* m = D->mvp;
+    const float x = dot4(m + 0, px, py, pz, 1.0f);
+    const float y = dot4(m + 4, px, py, pz, 1.0f);
+    const float z = dot4(m + 8, px, py, pz, 1.0f);
+    const float w = dot4(m + 12, px, py, pz, 1.0f);
+    const int na = kind_attrs(D->kind);
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    if (na >= 2) {
+        a0 = D->attr[na * i + 0];
+        a1 = D->attr[na * i + 1];
+    }
+    if (na == 3) a2 = D->attr[na * i + 2];
+    const uint32_t oc = outcode(x, y, z, w);
+    B.cv[slot] = make_float4(x, y, z, w);
+    B.ca[slot] = make_float4(a0, a1, a2, u2f(oc));
+    // to_screen (prim.rs:62-88): (x/w, y/w, 1/w), viewport, attribute z_div
+    const float nx = x / w, ny = y / w, nz = 1.0f / w;
+    const float* v = D->vp;
+    float* s = B.sv + 6 * slot;
+    s[0] = dot4(v + 0, nx, ny, nz, 1.0f);
+    s[1] = dot4(v + 4, nx, ny, nz, 1.0f);
+    s[2] = dot4(v + 8, nx, ny, nz, 1.0f);
+    const bool persp = kind_persp(D->kind);
+    s[3] = persp ? a0 / w : a0;
+    s[4] = persp ? a1 / w : a1;
+    s[5] = persp ? a2 / w : a2;
+}
+
+// ---------------------------------------------------------------------------
+// k_setup helpers: clipping in registers
+// ---------------------------------------------------------------------------
+struct CV {
+    float p[4];
+    float a[3];
+    uint32_t oc;
+};
+
+__device__ __forceinline__ void cv_make(CV& o, const float* p, const float* a) {
+    for (int k = 0; k < 4; ++k) o.p[k] = p[k];
+    for (int k = 0; k < 3; ++k) o.a[k] = a[k];
+    o.oc = outcode(p[0], p[1], p[2], p[3]);
+}
+
+// Screen vertex of a clip vertex (to_screen, prim.rs:62-88).
+__device__ __forceinline__ void to_screen(const CV& c, const float* vp, bool persp, float* s) {
+    const float w = c.p[3];
+    const float nx = c.p[0] / w, ny = c.p[1] / w, nz = 1.0f / w;
+    s[0] = dot4(vp + 0, nx, ny, nz, 1.0f);
+    s[1] = dot4(vp + 4, nx, ny, nz, 1.0f);
+    s[2] = dot4(vp + 8, nx, ny, nz, 1.0f);
+    s[3] = persp ? c.a[0] / w : c.a[0];
+    s[4] = persp ? c.a[1] / w : c.a[1];
+    s[5] = persp ? c.a[2] / w : c.a[2];
+}
+
+// Tri::winding() == Cw (geom/prim.rs:288-295).
+__device__ __forceinline__ bool backface(const float* s0, const float* s1, const float* s2) {
+    const float ux = s1[0] - s0[0], uy = s1[1] - s0[1];
+    const float vx = s2[0] - s0[0], vy = s2[1] - s0[1];
+    float z = 0.0f;
+    z = z + (-uy) * vx;
+    z = z + ux * vy;
+    return z < 0.0f;
+}
+__device__ __forceinline__ bool culled(int32_t fc, bool bf) {
+    return (fc == FC_BACK && bf) || (fc == FC_FRONT && !bf);
+}
+
+// Sutherland-Hodgman over the six planes (clip.rs:167-193, 283-301).
+// `poly` holds n vertices on entry; returns the output count (0..CLIP_MAX), or
+// -1 if a plane would produce more than CLIP_MAX vertices (a convex polygon
+// gains at most one vertex per plane, 3 + 6 = 9; more only arises from a
+// numerically non-convex polygon and is reported instead of stored).
+constexpr int CLIP_MAX = 9;
+__device__ int clip_polygon(CV* poly, int n, CV* tmp) {
+    CV* in = poly;
+    CV* out = tmp;
+    for (int p = 0; p < 6; ++p) {
+        const uint32_t bit = 1u << p;
+        int m = 0;
+        for (int k = 0; k < n; ++k) {
+            const CV& v0 = in[k];
+            const CV& v1 = in[k + 1 < n ? k + 1 : 0];
+            if ((bit & v0.oc) == 0) {
+                if (m == CLIP_MAX) return -1;
+                out[m++] = v0;
+            }
+            const float d0 = plane_dist(p, v0.p[0], v0.p[1], v0.p[2], v0.p[3]);
+            const float d1 = plane_dist(p, v1.p[0], v1.p[1], v1.p[2], v1.p[3]);
+            if (d0 * d1 < 0.0f) {
+                if (m == CLIP_MAX) return -1;
+                const float t = -d0 / (d1 - d0);
+                float q[4], a[3];
+                for (int c = 0; c < 4; ++c) q[c] = v0.p[c] + (v1.p[c] - v0.p[c]) * t;
+                for (int c = 0; c < 3; ++c) a[c] = v0.a[c] + (v1.a[c] - v0.a[c]) * t;
+                cv_make(out[m++], q, a);
+            }
+        }
+        n = m;
+        if (n == 0) break;
+        CV* s = in; in = out; out = s;
+    }
+    if (n > 0 && in != poly)
+        for (int k = 0; k < n; ++k) poly[k] = in[k];
+    return n;
+}
+
+__device__ __forceinline__ void write_scr(ScrTri* dst, const float* s0, const float* s1,
+                                          const float* s2, uint32_t draw) {
+    float* d = &dst->v[0][0];
+    for (int c = 0; c < 6; ++c) d[c] = s0[c];
+    for (int c = 0; c < 6; ++c) d[6 + c] = s1[c];
+    for (int c = 0; c < 6; ++c) d[12 + c] = s2[c];
+    dst->draw = draw;
+    dst->pad = 0;
+}
+
+struct RectAcc {
+    float xmin, xmax, ymin, ymax;
+    bool finite;
+    __device__ void init() {
+        xmin = ymin = __builtin_inff();
+        xmax = ymax = -__builtin_inff();
+        finite = true;
+    }
+    __device__ void add(const float* s) {
+        if (!(__builtin_isfinite(s[0]) && __builtin_isfinite(s[1]))) finite = false;
+        xmin = fminf(xmin, s[0]);
+        xmax = fmaxf(xmax, s[0]);
+        ymin = fminf(ymin, s[1]);
+        ymax = fmaxf(ymax, s[1]);
+    }
+};
+
+// Tile rectangle of a triangle's pixels. Rows are exact: tri_fill covers rows
+// floor(top_y + 0.5) .. floor(bot_y + 0.5) (scan, raster.rs:286-300). Columns
+// are the vertex x range widened by BIN_MARGIN pixels (stepped edges may drift
+// by rounding; the rasterizer checks every span stays inside).
+__device__ __forceinline__ bool tile_rect(const RectAcc& r, const FrameDesc& F, uint32_t tw,
+                                          uint32_t* out) {
+    const float r0 = floorf(r.ymin + 0.5f), r1 = floorf(r.ymax + 0.5f);
+    if (!(r1 > r0)) return false;
+    int64_t row0 = (int64_t)r0, row1 = (int64_t)r1 - 1;
+    int64_t c0 = (int64_t)floorf(r.xmin) - BIN_MARGIN, c1 = (int64_t)floorf(r.xmax) + BIN_MARGIN;
+    if (row0 < 0) row0 = 0;
+    if (c0 < 0) c0 = 0;
+    if (row1 >= (int64_t)F.height) row1 = (int64_t)F.height - 1;
+    if (c1 >= (int64_t)F.width) c1 = (int64_t)F.width - 1;
+    if (row1 < row0 || c1 < c0) return false;
+    const uint32_t tx0 = (uint32_t)(c0 / tw), tx1 = (uint32_t)(c1 / tw);
+    const uint32_t ty0 = (uint32_t)(row0 / TILE_H), ty1 = (uint32_t)(row1 / TILE_H);
+    out[0] = tx0 | (ty0 << 16);
+    out[1] = tx1 | (ty1 << 16);
+    return true;
+}
+
+// Visit, over the whole wave, each distinct tile in the union of the lanes'
+// rectangles, in increasing tile order. For every tile, `fn(tile, mask)` runs
+// on the lowest lane whose rectangle contains it; `mask` holds those lanes.
+// Each lane walks its own rectangle in row-major (= increasing id) order; the
+// wave minimum of the lanes' next tiles is processed next, so a lane whose
+// rectangle contains that tile always has it as its next one.
+template <class Fn>
+__device__ __forceinline__ void for_each_tile(bool valid, uint32_t r0, uint32_t r1, uint32_t base,
+                                              uint32_t ntx, Fn&& fn) {
+    uint32_t tx0 = r0 & 0xFFFF, ty0 = r0 >> 16, tx1 = r1 & 0xFFFF, ty1 = r1 >> 16;
+    uint32_t cx = tx0, cy = ty0;
+    while (true) {
+        const uint32_t next = valid ? base + cy * ntx + cx : 0xFFFFFFFFu;
+        const uint32_t t = wave_min_u32(next);
+        if (t == 0xFFFFFFFFu) break;
+        const bool mine = valid && next == t;
+        const unsigned long long m = __ballot(mine);
+        if (mine) {
+            if (lane_id() == (uint32_t)(__ffsll((long long)m) - 1)) fn(t, m);
+            if (++cx > tx1) {
+                cx = tx0;
+                if (++cy > ty1) valid = false;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_setup
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
+    const uint32_t lane = lane_id();
+    const uint32_t group = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (group >= B.n_groups) return;  // whole wave exits together
+    const uint64_t g = (uint64_t)group * WAVE + lane;
+    bool valid = g < B.n_tris;
+    uint32_t nf = 0, extra_base = 0;
+    uint32_t rect[2] = {0, 0};
+    bool has_rect = false;
+    uint32_t frame = 0xFFFFFFFFu;
+    if (valid) {
+        const uint32_t di = find_draw_tri(B.draws, B.n_draws, g);
+        const DrawDesc* D = &B.draws[di];
+        const FrameDesc& F = B.frames[D->frame];
+        frame = D->frame;
+        const uint64_t t = g - D->tri_base;
+        const uint32_t i0 = D->tris[3 * t], i1 = D->tris[3 * t + 1], i2 = D->tris[3 * t + 2];
+        if (i0 >= D->n_verts || i1 >= D->n_verts || i2 >= D->n_verts) {
+            atomicOr(B.err, ERRB_INDEX);
+        } else {
+            const uint64_t s0 = D->vert_base + i0, s1 = D->vert_base + i1, s2 = D->vert_base + i2;
+            const uint32_t o0 = f2u(B.ca[s0].w), o1 = f2u(B.ca[s1].w), o2 = f2u(B.ca[s2].w);
+            const uint32_t all_out = o0 & o1 & o2, any_out = o0 | o1 | o2;
+            RectAcc ra;
+            ra.init();
+            if (all_out != 0) {
+                // Hidden
+            } else if (any_out == 0) {
+                // Visible: screen vertices were computed per vertex
+                float v0[6], v1[6], v2[6];
+                for (int c = 0; c < 6; ++c) {
+                    v0[c] = B.sv[6 * s0 + c];
+                    v1[c] = B.sv[6 * s1 + c];
+                    v2[c] = B.sv[6 * s2 + c];
+                }
+                if (!culled(F.face_cull, backface(v0, v1, v2))) {
+                    write_scr(&B.scr[g], v0, v1, v2, di);
+                    ra.add(v0);
+                    ra.add(v1);
+                    ra.add(v2);
+                    nf = 1;
+                }
+            } else {
+                // Clipped
+                CV poly[CLIP_MAX], tmp[CLIP_MAX];
+                const uint64_t ss[3] = {s0, s1, s2};
+                for (int k = 0; k < 3; ++k) {
+                    const float4 p = B.cv[ss[k]];
+                    const float4 a = B.ca[ss[k]];
+                    const float pp[4] = {p.x, p.y, p.z, p.w};
+                    const float aa[3] = {a.x, a.y, a.z};
+                    cv_make(poly[k], pp, aa);
+                }
+                const int n = clip_polygon(poly, 3, tmp);
+                if (n < 0) atomicOr(B.err, ERRB_CLIP);
+                if (n >= 3) {
+                    const bool persp = kind_persp(D->kind);
+                    float sc[CLIP_MAX][6];
+                    for (int k = 0; k < n; ++k) to_screen(poly[k], D->vp, persp, sc[k]);
+                    // fan [p, e0, e1] over rest.windows(2) (clip.rs:373-396)
+                    uint32_t keep = 0, cnt = 0;
+                    for (int k = 1; k + 1 < n; ++k)
+                        if (!culled(F.face_cull, backface(sc[0], sc[k], sc[k + 1]))) {
+                            keep |= 1u << k;
+                            ++cnt;
+                        }
+                    if (cnt > 1) {
+                        extra_base = atomicAdd(B.extra_count, cnt - 1);
+                        if (extra_base + cnt - 1 > B.extra_cap) {
+                            atomicOr(B.err, ERRB_EXTRA);
+                            cnt = 0;
+                        }
+                    }
+                    uint32_t j = 0;
+                    for (int k = 1; k + 1 < n && j < cnt; ++k) {
+                        if (!(keep & (1u << k))) continue;
+                        ScrTri* dst = (j == 0) ? &B.scr[g] : &B.extra[extra_base + j - 1];
+                        write_scr(dst, sc[0], sc[k], sc[k + 1], di);
+                        ra.add(sc[0]);
+                        ra.add(sc[k]);
+                        ra.add(sc[k + 1]);
+                        ++j;
+                    }
+                    nf = cnt;
+                }
+            }
+            if (nf > 0) {
+                if (!ra.finite) {
+                    atomicOr(B.err, ERRB_NONFINITE);
+                    nf = 0;
+                } else if (ra.ymin < -0.5f) {
+                    // rows above -0.5 would map several scanlines onto row 0
+                    // through the saturating `as usize` (raster.rs:106)
+                    atomicOr(B.err, ERRB_NEGY);
+                    nf = 0;
+                } else {
+                    // rows at or below the last one panic in the reference
+                    // (target.rs:173-174); so does a span starting past the right edge
+                    const float rr0 = floorf(ra.ymin + 0.5f), rr1 = floorf(ra.ymax + 0.5f);
+                    if (rr1 > rr0 && (rr1 > (float)F.height || floorf(ra.xmin) > (float)F.width + 1.0f))
+                        atomicOr(B.err, ERRB_BOUNDS);
+                    has_rect = tile_rect(ra, F, B.tile_w, rect);
+                }
+            }
+        }
+    }
+    if (valid) {
+        B.hdr[g] = nf | (extra_base << 3);
+        B.rect[2 * g] = has_rect ? rect[0] : 0xFFFFFFFFu;
+        B.rect[2 * g + 1] = rect[1];
+    }
+    // prims.o / verts.o (render.rs:195-196): one atomic per wave when the
+    // wave's triangles all belong to one frame, else one per lane
+    {
+        const uint32_t fmin = wave_min_u32(nf > 0 ? frame : 0xFFFFFFFFu);
+        const uint32_t fmax = ~wave_min_u32(nf > 0 ? ~frame : 0xFFFFFFFFu);
+        if (fmin == fmax) {
+            const unsigned long long tot = wave_sum_u64(nf);
+            if (lane == 0 && tot) {
+                atomicAdd(&B.stats[fmin].prims_o, tot);
+                atomicAdd(&B.stats[fmin].verts_o, 3ull * tot);
+            }
+        } else if (nf > 0) {
+            atomicAdd(&B.stats[frame].prims_o, (unsigned long long)nf);
+            atomicAdd(&B.stats[frame].verts_o, 3ull * nf);
+        }
+    }
+    // per-tile run counts
+    const FrameDesc* F = has_rect ? &B.frames[frame] : nullptr;
+    const uint32_t base = has_rect ? F->tile_base : 0, ntx = has_rect ? F->ntx : 0;
+    for_each_tile(has_rect, rect[0], rect[1], base, ntx,
+                  [&](uint32_t tile, unsigned long long) { atomicAdd(&B.run_count[tile], 1u); });
+}
+
+// ---------------------------------------------------------------------------
+// k_scan: exclusive scan of run counts (one workgroup of 1024 threads)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_scan(BatchDesc B) {
+    __shared__ uint32_t warp_tot[16];
+    __shared__ uint32_t carry;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < B.n_tiles; base += 1024) {
+        const uint32_t i = base + tid;
+        const uint32_t v = i < B.n_tiles ? B.run_count[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = wave_excl_scan(v, &tot);
+        if (lane == 0) warp_tot[w] = tot;
+        __syncthreads();
+        uint32_t off = carry;
+        for (uint32_t k = 0; k < w; ++k) off += warp_tot[k];
+        if (i < B.n_tiles) {
+            B.run_off[i] = off + ex;
+            B.run_cursor[i] = 0;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t s = 0;
+            for (uint32_t k = 0; k < 16; ++k) s += warp_tot[k];
+            carry += s;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        B.run_off[B.n_tiles] = carry;
+        if (carry > B.runs_cap) atomicOr(B.err, ERRB_RUNS);
+        *B.tile_counter = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_bin: one run per (group, tile)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
+    const uint32_t lane = lane_id();
+    const uint32_t group = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (group >= B.n_groups) return;
+    const uint64_t g = (uint64_t)group * WAVE + lane;
+    bool has = false;
+    uint32_t r0 = 0, r1 = 0, base = 0, ntx = 0;
+    if (g < B.n_tris) {
+        r0 = B.rect[2 * g];
+        r1 = B.rect[2 * g + 1];
+        has = r0 != 0xFFFFFFFFu;
+        if (has) {
+            const uint32_t di = find_draw_tri(B.draws, B.n_draws, g);
+            const FrameDesc& F = B.frames[B.draws[di].frame];
+            base = F.tile_base;
+            ntx = F.ntx;
+        }
+    }
+    const uint32_t cap = B.runs_cap;
+    for_each_tile(has, r0, r1, base, ntx, [&](uint32_t tile, unsigned long long m) {
+        const uint32_t pos = B.run_off[tile] + atomicAdd(&B.run_cursor[tile], 1u);
+        if (pos < cap) {
+            Run r;
+            r.group = group;
+            r.pad = 0;
+            r.mask = m;
+            B.runs[pos] = r;
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// k_binsort: sort each tile's runs by group (bitonic, ascending)
+// ---------------------------------------------------------------------------
+constexpr int SORT_LDS = 4096;
+
+__global__ __launch_bounds__(256) void k_binsort(BatchDesc B) {
+    __shared__ unsigned long long key[SORT_LDS];
+    const uint32_t tile = blockIdx.x;
+    if (tile >= B.n_tiles) return;
+    const uint32_t off = B.run_off[tile];
+    uint32_t n = B.run_off[tile + 1] - off;
+    if (off >= B.runs_cap) return;
+    if (off + n > B.runs_cap) n = B.runs_cap - off;
+    if (n <= 1) return;
+    Run* runs = B.runs + off;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    // Is it already sorted? (common: one group per tile, or groups landing in order)
+    __shared__ int unsorted;
+    if (tid == 0) unsorted = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i + 1 < n; i += nt)
+        if (runs[i].group > runs[i + 1].group) unsorted = 1;
+    __syncthreads();
+    if (!unsorted) return;
+    uint32_t p2 = 1;
+    while (p2 < n) p2 <<= 1;
+    // Bitonic network in its all-ascending form: the first step of each merge
+    // compares i with i ^ (k - 1), later steps i with i ^ j, always putting
+    // the smaller key at the lower index. Entries at indices >= n act as +inf
+    // keys, so any pair whose upper index is >= n is a no-op and is skipped.
+    if (p2 <= SORT_LDS) {
+        // key = group << 32 | original index
+        for (uint32_t i = tid; i < n; i += nt) key[i] = ((unsigned long long)runs[i].group << 32) | i;
+        __syncthreads();
+        for (uint32_t k = 2; k <= p2; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t x = (j == (k >> 1)) ? (k - 1) : j;
+                for (uint32_t i = tid; i < n; i += nt) {
+                    const uint32_t l = i ^ x;
+                    if (l > i && l < n) {
+                        const unsigned long long a = key[i], b = key[l];
+                        if (a > b) {
+                            key[i] = b;
+                            key[l] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // gather masks in sorted order (read all first, then write)
+        unsigned long long m[SORT_LDS / 256];
+        uint32_t cnt = 0;
+        for (uint32_t i = tid; i < n; i += nt) m[cnt++] = runs[(uint32_t)key[i]].mask;
+        __syncthreads();
+        cnt = 0;
+        for (uint32_t i = tid; i < n; i += nt) {
+            Run r;
+            r.group = (uint32_t)(key[i] >> 32);
+            r.pad = 0;
+            r.mask = m[cnt++];
+            runs[i] = r;
+        }
+    } else {
+        // Large bin: the same network in place on the run records in global memory.
+        for (uint32_t k = 2; k <= p2; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t x = (j == (k >> 1)) ? (k - 1) : j;
+                for (uint32_t i = tid; i < n; i += nt) {
+                    const uint32_t l = i ^ x;
+                    if (l > i && l < n) {
+                        const Run a = runs[i], b = runs[l];
+                        if (a.group > b.group) {
+                            runs[i] = b;
+                            runs[l] = a;
+                        }
+                    }
+                }
+                __threadfence_block();
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_raster
+// ---------------------------------------------------------------------------
+struct Trap {
+    float y0i;             // first row (floor(y0 + 0.5))
+    uint32_t n;            // row count
+    float lx, lz, la[3];   // left state at the first row
+    float dlx, dlz, dla[3];
+    float rx, drx;         // right x at the first row, step
+    float dvz, dva[3];     // d(varyings)/dx
+};
+struct Slot {
+    Trap t[2];
+    int32_t ylo, yhi;      // rows covered (absolute)
+    uint32_t kind, flat;
+    const uint8_t* tex;
+    uint32_t tw, th, tstride;
+    uint32_t rx0, rx1;     // binned tile columns (margin check)
+};
+
+// scan (raster.rs:248-302) for one trapezoid; y components are not needed by
+// any output and are left out.
+__device__ __forceinline__ void scan_setup(Trap& T, float y0, float y1, const float* l0,
+                                           const float* l1, const float* r0, const float* r1,
+                                           int na) {
+    // l*/r* layout: x, y, z, a0, a1, a2
+    const float recip_dy = 1.0f / (y1 - y0);
+    float dl[6], dr[6];
+    for (int c = 0; c < 6; ++c) {
+        dl[c] = (l1[c] - l0[c]) * recip_dy;
+        dr[c] = (r1[c] - r0[c]) * recip_dy;
+    }
+    float l0s[6], r0s[6];
+    for (int c = 0; c < 6; ++c) {
+        l0s[c] = l0[c] + dl[c];
+        r0s[c] = r0[c] + dr[c];
+    }
+    const float dx = r0s[0] - l0s[0];
+    const float rdx = 1.0f / dx;
+    T.dvz = (r0s[2] - l0s[2]) * rdx;
+    for (int c = 0; c < 3; ++c) T.dva[c] = (c < na) ? (r0s[3 + c] - l0s[3 + c]) * rdx : 0.0f;
+    const float y0r = round_up_to_half(y0);
+    const float y1r = round_up_to_half(y1);
+    const float tweak = y0r - y0;
+    // l0.lerp(&l0.step(&dl_dy), y_tweak) == l0 + ((l0 + dl) - l0) * tweak
+    T.lx = l0[0] + (l0s[0] - l0[0]) * tweak;
+    T.lz = l0[2] + (l0s[2] - l0[2]) * tweak;
+    for (int c = 0; c < 3; ++c) T.la[c] = (c < na) ? l0[3 + c] + (l0s[3 + c] - l0[3 + c]) * tweak : 0.0f;
+    T.dlx = dl[0];
+    T.dlz = dl[2];
+    for (int c = 0; c < 3; ++c) T.dla[c] = (c < na) ? dl[3 + c] : 0.0f;
+    T.rx = r0[0] + dr[0] * tweak;
+    T.drx = dr[0];
+    T.n = as_u32(y1r - y0r);
+    T.y0i = y0r - 0.5f;
+}
+
+// tri_fill (raster.rs:185-224) setup for one screen triangle.
+__device__ __forceinline__ void tri_setup(Slot& S, const float* a, const float* b, const float* c,
+                                          int na) {
+    const float* v[3] = {a, b, c};
+    // stable sort by total_cmp of y
+    if (total_key(v[1][1]) < total_key(v[0][1])) { const float* t = v[0]; v[0] = v[1]; v[1] = t; }
+    if (total_key(v[2][1]) < total_key(v[1][1])) { const float* t = v[1]; v[1] = v[2]; v[2] = t; }
+    if (total_key(v[1][1]) < total_key(v[0][1])) { const float* t = v[0]; v[0] = v[1]; v[1] = t; }
+    const float* top = v[0];
+    const float* mid0 = v[1];
+    const float* bot = v[2];
+    const float top_y = top[1], mid_y = mid0[1], bot_y = bot[1];
+    const float tt = (mid_y - top_y) / (bot_y - top_y);
+    float mid1[6];
+    for (int k = 0; k < 6; ++k) mid1[k] = top[k] + (bot[k] - top[k]) * tt;
+    const bool l_is_mid0 = mid0[0] < mid1[0];
+    const float* left = l_is_mid0 ? mid0 : mid1;
+    const float* right = l_is_mid0 ? mid1 : mid0;
+    scan_setup(S.t[0], top_y, mid_y, top, left, top, right, na);
+    scan_setup(S.t[1], mid_y, bot_y, left, bot, right, bot, na);
+    int32_t lo = 0x7FFFFFFF, hi = -0x7FFFFFFF;
+    for (int h = 0; h < 2; ++h) {
+        if (S.t[h].n == 0) continue;
+        const int32_t y0 = (int32_t)S.t[h].y0i;
+        const int32_t y1 = y0 + (int32_t)S.t[h].n;
+        lo = y0 < lo ? y0 : lo;
+        hi = y1 > hi ? y1 : hi;
+    }
+    S.ylo = lo;
+    S.yhi = hi;
+}
+
+// Built-in fragment shader for one fragment, result in the target format.
+__device__ __forceinline__ uint32_t shade(const Slot& S, uint32_t fmt, float z, float a0, float a1,
+                                          float a2) {
+    uint32_t r, g, b, a;
+    switch (S.kind) {
+        case KIND_FLAT:
+            r = S.flat & 0xFF; g = (S.flat >> 8) & 0xFF; b = (S.flat >> 16) & 0xFF; a = S.flat >> 24;
+            break;
+        case KIND_GOURAUD:  // Color3f::to_color4 (color.rs:253-263)
+            r = as_u8(256.0f * a0); g = as_u8(256.0f * a1); b = as_u8(256.0f * a2); a = 0xFF;
+            break;
+        default: {
+            // Scanline::fragments z_div (raster.rs:61-68, vec.rs:666-676)
+            const float u = a0 / z, v = a1 / z;
+            const float tw = (float)S.tw, th = (float)S.th;
+            uint32_t ui, vi;
+            if (S.kind == KIND_TEX_CLAMP) {  // SamplerClamp (tex.rs:280-303)
+                float su = u * tw, sv = v * th;
+                const float mu = tw - 1.0f, mv = th - 1.0f;
+                if (su < 0.0f) su = 0.0f;
+                if (su > mu) su = mu;
+                if (sv < 0.0f) sv = 0.0f;
+                if (sv > mv) sv = mv;
+                ui = as_u32(floorf(su));
+                vi = as_u32(floorf(sv));
+            } else {  // SamplerRepeatPot (tex.rs:240-266)
+                ui = (uint32_t)as_i32(floorf(tw * u)) & (S.tw - 1);
+                vi = (uint32_t)as_i32(floorf(th * v)) & (S.th - 1);
+            }
+            const uint32_t t =
+                *(const uint32_t*)(S.tex + ((uint64_t)vi * S.tstride + ui) * 4ull);
+            r = t & 0xFF; g = (t >> 8) & 0xFF; b = (t >> 16) & 0xFF; a = t >> 24;
+        }
+    }
+    return pack_pixel(fmt, r, g, b, a);
+}
+
+__device__ __forceinline__ bool depth_pass(int32_t dt, float nz, float cur) {
+    switch (dt) {
+        case DT_NONE: return true;
+        case DT_LESS: return cur < nz;
+        case DT_EQUAL: return cur == nz;
+        default: return cur > nz;
+    }
+}
+
+__device__ __forceinline__ uint32_t load_pixel(const FrameDesc& F, uint32_t x, uint32_t y) {
+    const uint8_t* p = F.color + (uint64_t)y * F.color_stride;
+    if (F.fmt == PIX_RGB8) {
+        p += 3ull * x;
+        return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16);
+    }
+    return *(const uint32_t*)(p + 4ull * x);
+}
+__device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint32_t y, uint32_t v) {
+    uint8_t* p = F.color + (uint64_t)y * F.color_stride;
+    if (F.fmt == PIX_RGB8) {
+        p += 3ull * x;
+        p[0] = (uint8_t)v;
+        p[1] = (uint8_t)(v >> 8);
+        p[2] = (uint8_t)(v >> 16);
+        return;
+    }
+    *(uint32_t*)(p + 4ull * x) = v;
+}
+
+__device__ __forceinline__ uint32_t find_frame_tile(const FrameDesc* f, uint32_t n, uint32_t t) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (f[mid].tile_base <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <int TW>
+__global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
+    constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
+    __shared__ uint32_t s_col[TW * LD];
+    __shared__ float s_dep[TW * LD];
+    __shared__ Slot s_slot[WAVE];
+    __shared__ unsigned long long s_rowmask[TILE_H];
+    __shared__ uint16_t s_fan[WAVE * 8];
+    __shared__ uint32_t s_tile;
+
+    const uint32_t lane = lane_id();
+    for (;;) {
+        if (lane == 0) s_tile = atomicAdd(B.tile_counter, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        __syncthreads();
+        if (tile >= B.n_tiles) break;
+        const uint32_t fi = find_frame_tile(B.frames, B.n_frames, tile);
+        const FrameDesc F = B.frames[fi];
+        const uint32_t lt = tile - F.tile_base;
+        const uint32_t tx = lt % F.ntx, ty = lt / F.ntx;
+        const uint32_t x0 = tx * TW, y0 = ty * TILE_H;
+        const uint32_t off = B.run_off[tile];
+        uint32_t nruns = B.run_off[tile + 1] - off;
+        if (off >= B.runs_cap) nruns = 0;
+        else if (off + nruns > B.runs_cap) nruns = B.runs_cap - off;
+        const bool has_depth = F.depth != nullptr;
+        const uint32_t ncols = min((uint32_t)TW, F.width - x0);
+        const uint32_t nrows = min((uint32_t)TILE_H, F.height - y0);
+
+        if (nruns == 0) {
+            // Nothing drawn here: the fused clear is the only write.
+            if (F.clear_color)
+                for (uint32_t r = 0; r < nrows; ++r)
+                    for (uint32_t c = lane; c < ncols; c += WAVE) store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
+            if (has_depth && F.clear_depth)
+                for (uint32_t r = 0; r < nrows; ++r)
+                    for (uint32_t c = lane; c < ncols; c += WAVE)
+                        F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = F.clear_depth_value;
+            continue;
+        }
+
+        // tile init: fused clear or load from the target
+        for (uint32_t r = 0; r < nrows; ++r)
+            for (uint32_t c = lane; c < ncols; c += WAVE) {
+                s_col[c * LD + r] = F.clear_color ? F.clear_pixel : load_pixel(F, x0 + c, y0 + r);
+                if (has_depth)
+                    s_dep[c * LD + r] = F.clear_depth ? F.clear_depth_value
+                                                      : F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c];
+            }
+        s_rowmask[lane] = 0;
+        __syncthreads();
+
+        const int32_t row = (int32_t)(y0 + lane);
+        const bool row_in = lane < nrows;
+        unsigned long long fi_acc = 0, fo_acc = 0;
+        uint32_t errs = 0;
+
+        for (uint32_t ri = 0; ri < nruns; ++ri) {
+            const Run run = B.runs[off + ri];
+            const uint64_t g = (uint64_t)run.group * WAVE + lane;
+            const bool has = (run.mask >> lane) & 1ull;
+            const uint32_t h = has ? B.hdr[g] : 0u;
+            const uint32_t nf = h & 7u;
+            uint32_t F_total;
+            const uint32_t pre = wave_excl_scan(nf, &F_total);
+            for (uint32_t j = 0; j < nf; ++j) s_fan[pre + j] = (uint16_t)((lane << 3) | j);
+            __syncthreads();
+            for (uint32_t sub = 0; sub < F_total; sub += WAVE) {
+                const uint32_t idx = sub + lane;
+                if (idx < F_total) {
+                    const uint32_t e = s_fan[idx];
+                    const uint32_t li = e >> 3, j = e & 7u;
+                    const uint64_t gg = (uint64_t)run.group * WAVE + li;
+                    const ScrTri* st = (j == 0) ? &B.scr[gg] : &B.extra[(B.hdr[gg] >> 3) + j - 1];
+                    float v[3][6];
+                    for (int k = 0; k < 3; ++k)
+                        for (int c = 0; c < 6; ++c) v[k][c] = st->v[k][c];
+                    const DrawDesc* D = &B.draws[st->draw];
+                    Slot& S = s_slot[lane];
+                    const int na = kind_attrs(D->kind);
+                    tri_setup(S, v[0], v[1], v[2], na);
+                    S.kind = D->kind;
+                    S.flat = D->flat_rgba;
+                    S.tex = D->tex;
+                    S.tw = D->tex_w;
+                    S.th = D->tex_h;
+                    S.tstride = D->tex_stride;
+                    const uint32_t rr0 = B.rect[2 * gg], rr1 = B.rect[2 * gg + 1];
+                    S.rx0 = (rr0 & 0xFFFF) * TW;
+                    S.rx1 = ((rr1 & 0xFFFF) + 1) * TW;
+                    // mark the tile rows this fan covers
+                    int32_t a = S.ylo - (int32_t)y0, b = S.yhi - (int32_t)y0;
+                    a = a < 0 ? 0 : a;
+                    b = b > (int32_t)nrows ? (int32_t)nrows : b;
+                    for (int32_t r = a; r < b; ++r) atomicOr(&s_rowmask[r], 1ull << lane);
+                }
+                __syncthreads();
+                unsigned long long m = s_rowmask[lane];
+                s_rowmask[lane] = 0;
+                if (!row_in) m = 0;
+                while (m) {
+                    const uint32_t s = (uint32_t)(__ffsll((long long)m) - 1);
+                    m &= m - 1;
+                    const Slot& S = s_slot[s];
+                    const int na = kind_attrs(S.kind);
+                    for (int hh = 0; hh < 2; ++hh) {
+                        const Trap& T = S.t[hh];
+                        const int32_t k = row - (int32_t)T.y0i;
+                        if (k < 0 || (uint32_t)k >= T.n) continue;
+                        // ScanlineIter::next: left/right iterators stepped k times
+                        float lx = T.lx, lz = T.lz, l0 = T.la[0], l1 = T.la[1], l2 = T.la[2];
+                        float rx = T.rx;
+                        for (int32_t q = 0; q < k; ++q) {
+                            lx = lx + T.dlx;
+                            lz = lz + T.dlz;
+                            if (na > 0) {
+                                l0 = l0 + T.dla[0];
+                                l1 = l1 + T.dla[1];
+                                if (na > 2) l2 = l2 + T.dla[2];
+                            }
+                            rx = rx + T.drx;
+                        }
+                        const float rx0 = round_up_to_half(lx), rx1 = round_up_to_half(rx);
+                        // v0.lerp(&v0.step(&dv_dx), x0 - v0.x)
+                        const float tw_ = rx0 - lx;
+                        float z = lz + ((lz + T.dvz) - lz) * tw_;
+                        float c0 = l0 + ((l0 + T.dva[0]) - l0) * tw_;
+                        float c1 = l1 + ((l1 + T.dva[1]) - l1) * tw_;
+                        float c2 = l2 + ((l2 + T.dva[2]) - l2) * tw_;
+                        const uint32_t cnt = as_u32(rx1 - rx0);
+                        const uint64_t xs = as_usize(rx0);
+                        uint64_t xe = as_usize(rx1);
+                        if (xe < xs) xe = xs;
+                        if (xe > F.width || (uint32_t)row >= F.height) {
+                            errs |= ERRB_BOUNDS;
+                            continue;
+                        }
+                        if (xe > xs && (xs < S.rx0 || xe > S.rx1)) errs |= ERRB_MARGIN;
+                        // frags.i: the part of [xs, xe) inside this tile
+                        {
+                            const uint64_t a = xs > x0 ? xs : x0;
+                            const uint64_t b = xe < (uint64_t)(x0 + TW) ? xe : (uint64_t)(x0 + TW);
+                            if (b > a) fi_acc += b - a;
+                        }
+                        const uint64_t nfrag = (uint64_t)cnt < xe - xs ? (uint64_t)cnt : xe - xs;
+                        const int64_t j0 = (int64_t)x0 - (int64_t)xs > 0 ? (int64_t)x0 - (int64_t)xs : 0;
+                        int64_t j1 = (int64_t)x0 + TW - (int64_t)xs;
+                        if (j1 > (int64_t)nfrag) j1 = (int64_t)nfrag;
+                        if (j0 >= j1) continue;
+                        for (int64_t q = 0; q < j0; ++q) {
+                            z = z + T.dvz;
+                            if (na > 0) {
+                                c0 = c0 + T.dva[0];
+                                c1 = c1 + T.dva[1];
+                                if (na > 2) c2 = c2 + T.dva[2];
+                            }
+                        }
+                        const uint32_t lrow = lane;
+                        for (int64_t q = j0; q < j1; ++q) {
+                            const uint32_t col = (uint32_t)((int64_t)xs + q - (int64_t)x0);
+                            const uint32_t li = col * LD + lrow;
+                            if (has_depth) {
+                                const float cur = s_dep[li];
+                                if (depth_pass(F.depth_test, z, cur)) {
+                                    const uint32_t px = shade(S, F.fmt, z, c0, c1, c2);
+                                    if (F.color_write) {
+                                        ++fo_acc;
+                                        s_col[li] = px;
+                                    }
+                                    if (F.depth_write) s_dep[li] = z;
+                                }
+                            } else {
+                                const uint32_t px = shade(S, F.fmt, z, c0, c1, c2);
+                                if (F.color_write) {
+                                    ++fo_acc;
+                                    s_col[li] = px;
+                                }
+                            }
+                            z = z + T.dvz;
+                            if (na > 0) {
+                                c0 = c0 + T.dva[0];
+                                c1 = c1 + T.dva[1];
+                                if (na > 2) c2 = c2 + T.dva[2];
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // write the tile back once
+        for (uint32_t r = 0; r < nrows; ++r)
+            for (uint32_t c = lane; c < ncols; c += WAVE) {
+                store_pixel(F, x0 + c, y0 + r, s_col[c * LD + r]);
+                if (has_depth) F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = s_dep[c * LD + r];
+            }
+        const unsigned long long fi_s = wave_sum_u64(fi_acc), fo_s = wave_sum_u64(fo_acc);
+        if (lane == 0) {
+            if (fi_s) atomicAdd(&B.stats[fi].frags_i, fi_s);
+            if (fo_s) atomicAdd(&B.stats[fi].frags_o, fo_s);
+        }
+        uint32_t e = errs;
+        for (int o = 32; o > 0; o >>= 1) e |= __shfl_xor(e, o, WAVE);
+        if (lane == 0 && e) atomicOr(B.err, e);
+        __syncthreads();
+    }
+}
+
+// Zeroes per-batch counters (a kernel, so it can live inside a captured graph).
+__global__ void k_reset(BatchDesc B) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < B.n_tiles) B.run_count[i] = 0;
+    if (i < B.n_frames) {
+        B.stats[i].prims_o = 0;
+        B.stats[i].verts_o = 0;
+        B.stats[i].frags_i = 0;
+        B.stats[i].frags_o = 0;
+    }
+    if (i == 0) {
+        *B.extra_count = 0;
+        *B.err = 0;
+        *B.tile_counter = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+static hipError_t launch_impl(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev) {
+    const uint32_t reset_n = B.n_tiles > B.n_frames ? B.n_tiles : B.n_frames;
+    auto mark = [&](int i) {
+        if (ev) (void)hipEventRecord(ev[i], s);
+    };
+    mark(0);
+    hipLaunchKernelGGL(k_reset, dim3((reset_n + 255) / 256 + 1), dim3(256), 0, s, B);
+    if (B.n_verts)
+        hipLaunchKernelGGL(k_vertex, dim3((uint32_t)((B.n_verts + 255) / 256)), dim3(256), 0, s, B);
+    mark(1);
+    if (B.n_groups) hipLaunchKernelGGL(k_setup, dim3((B.n_groups + 3) / 4), dim3(256), 0, s, B);
+    mark(2);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B);
+    mark(3);
+    if (B.n_groups) hipLaunchKernelGGL(k_bin, dim3((B.n_groups + 3) / 4), dim3(256), 0, s, B);
+    mark(4);
+    hipLaunchKernelGGL(k_binsort, dim3(B.n_tiles), dim3(256), 0, s, B);
+    mark(5);
+    if (B.tile_w == 32)
+        hipLaunchKernelGGL(k_raster<32>, dim3(raster_grid), dim3(64), 0, s, B);
+    else
+        hipLaunchKernelGGL(k_raster<64>, dim3(raster_grid), dim3(64), 0, s, B);
+    mark(6);
+    return hipGetLastError();
+}
+
+hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid) {
+    return launch_impl(B, s, raster_grid, nullptr);
+}
+hipError_t launch_pipeline_timed(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev) {
+    return launch_impl(B, s, raster_grid, ev);
+}
+
+}  // namespace rf

@@ -1,0 +1,133 @@
+// Device-side descriptors shared by the host runtime and the kernels.
+//
+// A *batch* is one or more frames; a frame is one render target, one context
+// and an ordered list of draws; a draw is one `render()` call of the
+// reference (core/src/render.rs:134-207): one mesh, one built-in shader kind,
+// one uniform (the mvp matrix) and one `to_screen` matrix. Primitive
+// submission order -- which decides depth-test ties -- is draw order, then
+// triangle order within the draw, then clip-fan order (clip.rs:360-398).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rf {
+
+// Shader kinds (the built-in vertex/fragment shader pairs).
+enum ShaderKind : uint32_t {
+    KIND_FLAT = 0,            // constant Color4, varying ()
+    KIND_GOURAUD = 1,         // Color3f varying -> Color3f::to_color4
+    KIND_TEX_CLAMP = 2,       // TexCoord varying (z-divided) -> SamplerClamp
+    KIND_TEX_REPEAT_POT = 3,  // TexCoord varying (z-divided) -> SamplerRepeatPot
+};
+
+__host__ __device__ inline int kind_attrs(uint32_t k) {
+    return k == KIND_FLAT ? 0 : (k == KIND_GOURAUD ? 3 : 2);
+}
+__host__ __device__ inline bool kind_persp(uint32_t k) { return k >= KIND_TEX_CLAMP; }
+
+// Colour buffer formats.
+enum PixFmt : uint32_t {
+    PIX_RGBA8 = 0,      // [u8; 4] r, g, b, a  (Buf2<Color4>, Rgba8888 bytes)
+    PIX_RGB8 = 1,       // [u8; 3] r, g, b     (Buf2<Color3>)
+    PIX_U32_RGBA = 2,   // u32 0xRRGGBBAA      (Rgba8888 into u32)
+    PIX_U32_XRGB = 3,   // u32 0x00RRGGBB      (Xrgb8888)
+};
+
+enum DepthTest : int32_t { DT_NONE = 0, DT_LESS = 1, DT_EQUAL = 2, DT_GREATER = 3 };
+enum FaceCull : int32_t { FC_NONE = 0, FC_BACK = 1, FC_FRONT = 2 };
+
+// Error bits raised by kernels (OR-ed into one device word per batch).
+enum ErrBits : uint32_t {
+    ERRB_INDEX = 1u << 0,      // a vertex index >= vertex count (prim.rs:17-19 panics)
+    ERRB_BOUNDS = 1u << 1,     // a span outside the target (target.rs:173-174 panics)
+    ERRB_MARGIN = 1u << 2,     // a span left the tiles its triangle was binned to
+    ERRB_EXTRA = 1u << 3,      // clip-fan storage exhausted
+    ERRB_RUNS = 1u << 4,       // bin-run storage exhausted
+    ERRB_NONFINITE = 1u << 5,  // a screen-space vertex is NaN or infinite
+    ERRB_NEGY = 1u << 6,       // a screen-space vertex lies above row -0.5
+    ERRB_CLIP = 1u << 7,       // clipping produced more than 9 vertices
+};
+
+struct DrawDesc {
+    const float* pos;       // n_verts x (x, y, z)
+    const float* attr;      // n_verts x kind_attrs(kind)
+    const uint32_t* tris;   // n_tris x (i0, i1, i2)
+    const uint8_t* tex;     // Color4 texels (textured kinds)
+    uint32_t n_verts, n_tris;
+    uint32_t kind, frame;
+    uint32_t tex_w, tex_h, tex_stride;  // stride in texels
+    uint32_t flat_rgba;                 // r | g << 8 | b << 16 | a << 24
+    uint64_t vert_base;                 // first slot in the batch vertex buffers
+    uint64_t tri_base;                  // first global triangle index
+    float mvp[16];                      // row-major ProjMat3 (the uniform)
+    float vp[16];                       // row-major Mat4<Ndc, Screen>
+};
+
+struct FrameDesc {
+    uint8_t* color;          // device colour buffer
+    float* depth;            // device depth buffer or null (colour-only target)
+    uint64_t color_stride;   // bytes per row
+    uint64_t depth_stride;   // floats per row
+    uint32_t width, height, fmt;
+    int32_t depth_test, face_cull, color_write, depth_write;
+    int32_t clear_color, clear_depth;  // fused Frame::clear (front/src/lib.rs:103-120)
+    uint32_t clear_pixel;              // clear colour already in the target format
+    float clear_depth_value;           // 1 / depth_clear
+    uint32_t ntx, nty;                 // tile grid
+    uint32_t tile_base;                // first global tile index of this frame
+    uint32_t pad;
+};
+
+// Per-frame statistics (stats.rs:16-40), accumulated by the kernels.
+struct FrameStats {
+    unsigned long long prims_o, verts_o, frags_i, frags_o;
+};
+
+// Screen-space triangle as stored by the setup kernel: three vertices of
+// (x, y, z, a0, a1, a2) plus the draw index.
+struct ScrTri {
+    float v[3][6];
+    uint32_t draw;
+    uint32_t pad;
+};
+
+// A bin run: the triangles of group `group` (64 consecutive global triangle
+// indices) that touch one tile, as a bit mask.
+struct Run {
+    uint32_t group;
+    uint32_t pad;
+    unsigned long long mask;
+};
+
+struct BatchDesc {
+    const DrawDesc* draws;
+    const FrameDesc* frames;
+    FrameStats* stats;
+    uint32_t n_draws, n_frames;
+    uint64_t n_verts;   // total vertex slots
+    uint64_t n_tris;    // total input triangles
+    uint32_t n_groups;  // ceil(n_tris / 64)
+    uint32_t n_tiles;   // total tiles over all frames
+    uint32_t tile_w, tile_h;
+    // scratch
+    float4* cv;          // clip-space position per vertex slot
+    float4* ca;          // clip attributes (a0, a1, a2, outcode bits)
+    float* sv;           // screen vertex (x, y, z, a0, a1, a2) per slot
+    uint32_t* hdr;       // per triangle: surviving fan count | extra base << 3
+    ScrTri* scr;         // per triangle: first surviving fan
+    ScrTri* extra;       // further fans of clipped triangles
+    uint32_t* extra_count;
+    uint32_t extra_cap;
+    uint32_t* rect;      // per triangle: tile rect (tx0 | ty0 << 16, tx1 | ty1 << 16), 2 words
+    uint32_t* run_count; // per tile
+    uint32_t* run_off;   // per tile + 1 (exclusive scan)
+    uint32_t* run_cursor;
+    Run* runs;
+    uint32_t runs_cap;
+    uint32_t* tile_counter;  // persistent raster work counter
+    uint32_t* err;
+};
+
+}  // namespace rf

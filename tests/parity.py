@@ -1,0 +1,88 @@
+"""Helpers shared by the GPU parity tests: run one render() call on the
+oracle and on the HIP path from identical inputs and compare."""
+import numpy as np
+
+import oracle_lib as O
+import retrofire_amd as rf
+
+DT = {None: 0, "less": 1, "equal": 2, "greater": 3}
+FC = {None: 0, "back": 1, "front": 2}
+
+
+def make_target(w, h, fmt, with_depth, seed=0, clear=None):
+    rng = np.random.default_rng(seed)
+    if fmt in (rf.PIX_U32_RGBA, rf.PIX_U32_XRGB):
+        col = rng.integers(0, 2 ** 32, size=(h, w), dtype=np.uint32)
+    else:
+        col = rng.integers(0, 256, size=(h, w, 3 if fmt == rf.PIX_RGB8 else 4), dtype=np.uint8)
+    dep = None
+    if with_depth:
+        dep = rng.uniform(0.0, 0.02, size=(h, w)).astype(np.float32)
+    return col, dep
+
+
+def pack_clear(fmt, c):
+    r, g, b, a = c
+    if fmt == rf.PIX_RGBA8:
+        return np.array([r, g, b, a], np.uint8)
+    if fmt == rf.PIX_RGB8:
+        return np.array([r, g, b], np.uint8)
+    if fmt == rf.PIX_U32_RGBA:
+        return np.uint32((r << 24) | (g << 16) | (b << 8) | a)
+    return np.uint32((r << 16) | (g << 8) | b)
+
+
+def run_both(kind, tris, pos, attr, mvp, vp, w, h, fmt=rf.PIX_RGBA8, with_depth=True, flat=(255, 255, 255, 255),
+             tex=None, ctx=None, seed=0, calls=1):
+    """Renders `calls` times into identical targets on both paths; returns
+    (oracle colour, oracle depth, oracle stats), (gpu colour, gpu depth, gpu stats)."""
+    ctx = ctx or rf.Context()
+    col, dep = make_target(w, h, fmt, with_depth, seed)
+    ocol, odep = col.copy(), (dep.copy() if dep is not None else None)
+    gcol, gdep = col.copy(), (dep.copy() if dep is not None else None)
+    if ctx.clear:
+        if ctx.color_clear is not None:
+            ocol[...] = pack_clear(fmt, ctx.color_clear)
+        if odep is not None and ctx.depth_clear is not None:
+            odep[...] = np.float32(1.0) / np.float32(ctx.depth_clear)
+    ost = np.zeros(8, np.uint64)
+    octx = (DT[ctx.depth_test], FC[ctx.face_cull], int(ctx.color_write), int(ctx.depth_write))
+    if kind == 0:
+        shd = rf.shader.Flat(flat)
+    elif kind == 1:
+        shd = rf.shader.Gouraud()
+    elif kind == 2:
+        shd = rf.shader.TexClamp(tex)
+    else:
+        shd = rf.shader.TexRepeatPot(tex)
+    gctx = rf.Context(color_clear=ctx.color_clear, depth_clear=ctx.depth_clear, face_cull=ctx.face_cull,
+                      depth_test=ctx.depth_test, color_write=ctx.color_write, depth_write=ctx.depth_write,
+                      clear=ctx.clear)
+    target = rf.Framebuf(gcol, gdep, fmt) if with_depth else rf.Colorbuf(gcol, fmt)
+    verts = (pos, attr) if attr is not None else pos
+    for i in range(calls):
+        err, _ = O.render(kind, tris, pos, attr, mvp, vp, ocol, depth=odep, fmt=fmt, flat=flat, tex=tex,
+                          ctx=octx, stats=ost)
+        assert err == 0, f"oracle error {err}"
+        rf.render(tris, verts, shd, mvp, vp, target, gctx)
+        gctx.clear = False  # a frame clears once
+    gst = gctx.stats
+    g = np.array([gst.calls, gst.prims_i, gst.prims_o, gst.verts_i, gst.verts_o, gst.frags_i, gst.frags_o, 0],
+                 np.uint64)
+    return (ocol, odep, ost), (gcol, gdep, g)
+
+
+def assert_same(o, g, color_tol=0):
+    ocol, odep, ost = o
+    gcol, gdep, gst = g
+    if odep is not None:
+        ob, gb = odep.view(np.uint32), gdep.view(np.uint32)
+        bad = np.argwhere(ob != gb)
+        assert bad.shape[0] == 0, f"depth differs at {bad.shape[0]} px, first {bad[:5].tolist()}"
+    if color_tol == 0:
+        bad = np.argwhere(ocol != gcol)
+        assert bad.shape[0] == 0, f"colour differs at {bad.shape[0]} entries, first {bad[:5].tolist()}"
+    else:
+        d = np.abs(ocol.astype(np.int64) - gcol.astype(np.int64))
+        assert d.max() <= color_tol
+    assert ost[:7].tolist() == gst[:7].tolist(), f"stats oracle {ost[:7].tolist()} gpu {gst[:7].tolist()}"

@@ -1,0 +1,158 @@
+"""Parity of the HIP path with the oracle on the same seeded inputs: colour
+bytes, depth bits and Stats must be identical (the tolerance the brief allows
+for colour is +-1 LSB per 8-bit channel; the tests use 0)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import retrofire_amd as rf
+from parity import assert_same, run_both
+from retrofire_amd import mat as M
+from retrofire_amd import scenes as S
+from test_oracle_kats import GOLDEN, hello_tri_inputs, read_ppm, textured_quad_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def test_textured_quad_golden_gpu():  # core/tests/rendering.rs:18-60
+    verts, uvs, faces, tex, mvp, vp = textured_quad_inputs()
+    fb = np.zeros((256, 256, 3), np.uint8)
+    ctx = rf.Context()
+    rf.render(faces, (verts, uvs), rf.shader.TexClamp(tex), mvp, vp, rf.Colorbuf(fb), ctx)
+    np.testing.assert_array_equal(fb, read_ppm(os.path.join(GOLDEN, "textured_quad.ppm")))
+    assert ctx.stats.frags_i == 256 * 256 and ctx.stats.prims_o == 2
+
+
+def test_hello_tri_gpu():  # core/examples/hello_tri.rs:47-53
+    verts, cols, faces, mvp, vp = hello_tri_inputs()
+    fb = np.zeros((480, 640, 4), np.uint8)
+    rf.render(faces, (verts, cols), rf.shader.Gouraud(), mvp, vp, rf.Colorbuf(fb), rf.Context())
+    assert tuple(fb[240, 320]) == (114, 102, 128, 255)
+    gold = read_ppm(os.path.join(GOLDEN, "triangle.ppm"))
+    np.testing.assert_array_equal(fb[..., 3] == 255, gold.any(axis=2))
+
+
+def _scene_parity(sc, ctx=None, with_depth=True, fmt=rf.PIX_RGBA8):
+    ctx = ctx or rf.Context(clear=True)
+    o, g = run_both(sc.kind, sc.tris, sc.pos, sc.attr, sc.mvp, sc.viewport, sc.width, sc.height, fmt=fmt,
+                    with_depth=with_depth, tex=sc.tex, flat=sc.flat, ctx=ctx)
+    assert_same(o, g)
+    return o, g
+
+
+def test_sphere640_parity():
+    o, g = _scene_parity(S.sphere640())
+    assert g[2][6] > 50000  # the visible hemisphere is drawn
+
+
+def test_crates1080_parity():
+    o, g = _scene_parity(S.crates1080())
+    assert g[2][6] > 100000
+
+
+def test_icospheres_parity_reduced():
+    sc = S.icospheres1080(n=1500)
+    o, g = _scene_parity(sc)
+    assert g[2][6] > 10000
+
+
+@pytest.mark.parametrize("fmt", [rf.PIX_RGBA8, rf.PIX_RGB8, rf.PIX_U32_RGBA, rf.PIX_U32_XRGB])
+def test_pixel_formats(fmt):
+    _scene_parity(S.sphere640(), fmt=fmt, ctx=rf.Context(clear=True))
+    _scene_parity(S.sphere640(), fmt=fmt, ctx=rf.Context(clear=False), with_depth=False)
+
+
+CTXS = [
+    dict(depth_test="less", face_cull="back"),
+    dict(depth_test=None, face_cull=None),
+    dict(depth_test="greater", face_cull="front"),
+    dict(depth_test="equal", face_cull=None),
+    dict(depth_test="less", face_cull=None, color_write=False),
+    dict(depth_test="less", face_cull=None, depth_write=False),
+]
+
+
+def random_soup(rng, n, w, h, kind):
+    """Random overlapping triangles in front of and across the camera (some
+    crossing the near plane, some outside the frustum)."""
+    pos = rng.uniform([-3, -2, -1.5], [3, 2, 6], size=(3 * n, 3)).astype(np.float32)
+    tris = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    attr = None
+    if kind == 1:
+        attr = rng.uniform(0, 1, size=(3 * n, 3)).astype(np.float32)
+    elif kind >= 2:
+        attr = rng.uniform(-1.5, 2.5, size=(3 * n, 2)).astype(np.float32)
+    mvp = M.then(M.translate3(0, 0, 2.0), M.perspective(1.2, np.float32(w) / np.float32(h), 0.1, 8.0))
+    vp = M.viewport(0, h, w, 0)
+    return pos, attr, tris, mvp, vp
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("ci", range(len(CTXS)))
+def test_random_soup_contexts(kind, ci):
+    rng = np.random.default_rng(100 * kind + ci)
+    w, h = 301, 197
+    pos, attr, tris, mvp, vp = random_soup(rng, 300, w, h, kind)
+    tex = None
+    if kind >= 2:
+        tw = 64 if kind == 3 else 37
+        tex = rng.integers(0, 256, size=(32, tw, 4), dtype=np.uint8)
+    ctx = rf.Context(clear=(ci % 2 == 0), **CTXS[ci])
+    o, g = run_both(kind, tris, pos, attr, mvp, vp, w, h, tex=tex, flat=(9, 200, 77, 255), ctx=ctx, seed=ci,
+                    calls=2)
+    assert_same(o, g)
+
+
+def test_many_calls_same_target_ties():
+    """Several render() calls into one target: equal depths keep the first
+    drawn fragment (ctx.rs:86-89)."""
+    sc = S.sphere640()
+    o, g = run_both(sc.kind, sc.tris, sc.pos, sc.attr, sc.mvp, sc.viewport, sc.width, sc.height,
+                    ctx=rf.Context(clear=True), calls=3)
+    assert_same(o, g)
+
+
+def test_viewport_offset_and_small_target():
+    sc = S.crates1080(n_crates=300)
+    w, h = 500, 300
+    vp = M.viewport(10, h - 10, w - 10, 10)  # as the crates demo (crates.rs:53)
+    o, g = run_both(sc.kind, sc.tris, sc.pos, sc.attr, sc.mvp, vp, w, h, tex=sc.tex, ctx=rf.Context(clear=True))
+    assert_same(o, g)
+
+
+def test_empty_and_degenerate():
+    w, h = 64, 48
+    vp = M.viewport(0, h, w, 0)
+    mvp = M.then(M.translate3(0, 0, 3), M.perspective(1.0, 1.0, 0.1, 10.0))
+    pos = np.array([[0, 0, 0], [0, 0, 0], [0, 0, 0], [1, 1, 0], [1, 1, 0], [0.5, 0.2, 0], [-1, -1, 0],
+                    [1, -1, 0], [0, 1, 0]], np.float32)
+    tris = np.array([[0, 1, 2], [3, 4, 5], [6, 7, 8], [6, 8, 7]], np.uint32)
+    o, g = run_both(0, tris, pos, None, mvp, vp, w, h, ctx=rf.Context(clear=True, face_cull=None))
+    assert_same(o, g)
+    o, g = run_both(0, tris[:0], pos, None, mvp, vp, w, h, ctx=rf.Context(clear=True))
+    assert_same(o, g)
+
+
+def test_errors():
+    fb = np.zeros((8, 8, 4), np.uint8)
+    pos = np.zeros((3, 3), np.float32)
+    with pytest.raises(rf.RenderError) as e:
+        rf.render(np.array([[0, 1, 3]]), pos, rf.shader.Flat(), M.identity(), M.viewport(0, 0, 8, 8),
+                  rf.Colorbuf(fb), rf.Context())
+    assert e.value.status == -1
+    tex = np.zeros((3, 5, 4), np.uint8)
+    with pytest.raises(rf.RenderError) as e:
+        rf.render(np.array([[0, 1, 2]]), (pos, np.zeros((3, 2), np.float32)), rf.shader.TexRepeatPot(tex),
+                  M.identity(), M.viewport(0, 0, 8, 8), rf.Colorbuf(fb), rf.Context())
+    assert e.value.status == -3
+    with pytest.raises(rf.RenderError) as e:
+        rf.render(np.array([[0, 1, 2]]), pos, rf.shader.Flat(), M.identity(), M.viewport(0, 0, 8, 8),
+                  rf.Colorbuf(fb), rf.Context(depth_sort="back_to_front"))
+    assert e.value.status == -5
+    verts = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32)
+    with pytest.raises(rf.RenderError) as e:  # viewport larger than the target (target.rs:173 panics)
+        rf.render(np.array([[0, 1, 2]]), verts, rf.shader.Flat(), M.identity(), M.viewport(0, 0, 16, 16),
+                  rf.Colorbuf(fb), rf.Context(face_cull=None))
+    assert e.value.status == -2
