@@ -1,0 +1,257 @@
+"""Benchmark of the render path on one MI355X.
+
+Workload (the metric's 1080p textured case): the crates scene -- 1000
+textured unit cubes on a 100 x 100 field (one merged mesh, 24,000 vertices,
+12,000 triangles, one 256 x 256 RGBA8 texture, SamplerRepeatPot) -- rendered
+at 1920x1080 into an RGBA8 colour buffer and an f32 depth buffer, cleared per
+frame, from a first-person camera at height 1.7 (near-plane clipping). One
+step is one launch of the batched path over 64 such frames, each with its own
+camera heading; meshes, texture and targets are resident in HBM before the
+timed region.
+
+Printed as one JSON line: frames/s (value), Mfrag/s, the plain per-frame call
+rate, serving p50/p99 latency, the roofline of the rasterizer and the CPU
+baseline (the oracle, a single-threaded C++ restatement of the reference CPU
+path, on a bounded sample).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--frames", type=int, default=64)
+    p.add_argument("--tile-w", type=int, default=0, help="0 = autotune")
+    p.add_argument("--serve-seconds", type=float, default=2.0)
+    p.add_argument("--serve-rate", type=float, default=1000.0)
+    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--no-plain", action="store_true")
+    return p.parse_args()
+
+
+def camera_frames(n, w, h, seed=12345):
+    """n view-projection matrices: first person at (0, 1.7, 0), headings from
+    a seeded generator."""
+    from retrofire_amd import mat as M
+    from retrofire_amd import scenes as S
+    rng = S.Xorshift64(S.DEFAULT_SEED + 7 + seed)
+    aspect = np.float32(w) / np.float32(h)
+    proj = M.perspective(M.fov_diagonal_focal_ratio(90.0, aspect), aspect, 0.1, 1000.0)
+    mvps = []
+    for i in range(n):
+        yaw = float(rng.uniform_f32(0.0, 1.0)) if i else 0.0
+        mvps.append(M.then(S.first_person(0.0, 1.7, 0.0, yaw), proj))
+    return mvps, M.viewport(0, h, w, 0)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+    import retrofire_amd as rf
+    from retrofire_amd import batch as RB
+    from retrofire_amd import scenes as S
+
+    W, H = 1920, 1080
+    pos, uv, tris, tex = S.crates_world(1000)
+    dev = rf.default_device()
+    mesh = RB.Mesh(pos, tris, uv)
+    texr = RB.Texture(tex)
+    shd = rf.shader.TexRepeatPot(tex)
+    mvps, vp = camera_frames(a.frames, W, H)
+    fbs = [RB.DeviceFramebuf(W, H, rf.PIX_RGBA8, True) for _ in range(a.frames)]
+    frames = [(fbs[i], rf.Context(clear=True), [RB.mesh_draw(mesh, shd, mvps[i], vp, texture=texr)])
+              for i in range(a.frames)]
+    batch = RB.Batch(frames, tile_w=a.tile_w)
+
+    for _ in range(a.warmup):
+        batch.run(sync=True)
+    stats = batch.wait()
+    frags_i = sum(s.frags_i for s in stats)
+    frags_o = sum(s.frags_o for s in stats)
+    batch.run(sync=True)  # make sure the device is idle and synchronized
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        batch.run(sync=False)
+    batch.wait()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / a.steps
+    fps = a.frames * a.steps / elapsed
+    mfrag = frags_i * a.steps / elapsed / 1e6
+
+    # per-kernel durations (HIP events on the batch stream, eager launches)
+    prof = batch.profile(iters=max(3, min(10, a.steps)))
+    names = ["vertex", "setup", "scan", "bin", "binsort", "raster"]
+    raster_ms = prof[5]
+    bytes_per_frame_raster = W * H * (4 + 4) + tex.nbytes  # colour + depth written once, texture once
+    alg_bytes = bytes_per_frame_raster * a.frames
+    achieved = alg_bytes / (raster_ms * 1e-3) / 1e9
+    total_prof_ms = sum(prof)
+    alg_frame_bytes = W * H * 8 + pos.nbytes + uv.nbytes + tris.nbytes + tex.nbytes
+
+    out = {
+        "metric": "Mfrag/s + frames/s @1080p textured; p50/p99 latency, batched vs plain",
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded crates scene, resident in HBM)",
+        "config": {"workload": "crates1080: 1000 textured crates (24k verts, 12k tris, 256x256 RGBA8, "
+                               "SamplerRepeatPot), 1920x1080 RGBA8 + f32 depth, clear per frame",
+                   "frames_per_step": a.frames, "tile_w": batch.tile_w, "tile_h": 64,
+                   "parallelism": "single GPU"},
+        "mfrag_per_s": round(mfrag, 1),
+        "frags_i_per_frame": frags_i // a.frames,
+        "frags_o_per_frame": frags_o // a.frames,
+        "stage_ms": {n: round(x, 4) for n, x in zip(names, prof)},
+        "roofline": {"bound": "hbm", "kernel": "k_raster", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "alg_bytes_per_launch": alg_bytes, "launch_ms": round(raster_ms, 4)},
+        "pipeline_alg_GBs": round(alg_frame_bytes * a.frames / (total_prof_ms * 1e-3) / 1e9, 1),
+    }
+
+    # plain drop-in call: host buffers, upload + run + readback per frame
+    if not a.no_plain:
+        col = np.zeros((H, W, 4), np.uint8)
+        dep = np.zeros((H, W), np.float32)
+        ctx = rf.Context(clear=True)
+        lat = []
+        for i in range(8):
+            t = time.perf_counter()
+            rf.render(tris, (pos, uv), shd, mvps[i % a.frames], vp, rf.Framebuf(col, dep), ctx)
+            lat.append(time.perf_counter() - t)
+        lat = sorted(lat[2:])
+        out["plain"] = {"p50_ms": round(1e3 * lat[len(lat) // 2], 3), "frames_per_s": round(1.0 / lat[len(lat) // 2], 1),
+                        "note": "one synchronous rf_render per frame incl. host->device upload and readback"}
+        out["batched_over_plain"] = round(fps * lat[len(lat) // 2], 1)
+
+    # serving: open-loop Poisson arrivals over resident meshes
+    if not a.no_serve and a.serve_seconds > 0:
+        out["serving"] = serve_run(rf, RB, mesh, texr, shd, a)
+
+    # CPU baseline: the oracle on the same workload, one thread, bounded sample
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(pos, uv, tris, tex, mvps, vp, W, H, a.cpu_seconds)
+        out["gpu_over_cpu"] = round(fps / out["cpu_baseline"]["value"], 1)
+
+    batch.free()
+    for fb in fbs:
+        fb.free()
+    print(json.dumps(out), flush=True)
+
+
+def serve_run(rf, RB, mesh, texr, shd, a):
+    from retrofire_amd import mat as M
+    from retrofire_amd import scenes as S
+    rng = S.Xorshift64(S.DEFAULT_SEED + 4)
+    res_list = [(256, 256), (512, 512), (1280, 720), (1920, 1080), (2048, 2048)]
+    srv = RB.Server(budget_bytes=4 << 30, max_batch=32, n_streams=3)
+    n = max(1, int(a.serve_rate * a.serve_seconds))
+    reqs = []
+    t = 0.0
+    for i in range(n):
+        u = float(rng.uniform_f32(0.0, 1.0))
+        t += -math.log(max(1e-7, 1.0 - u)) / a.serve_rate
+        w, h = res_list[rng.uniform_usize(0, 5)]
+        yaw = float(rng.uniform_f32(0.0, 1.0))
+        aspect = np.float32(w) / np.float32(h)
+        proj = M.perspective(M.fov_diagonal_focal_ratio(90.0, aspect), aspect, 0.1, 1000.0)
+        mvp = M.then(S.first_person(0.0, 1.7, 0.0, yaw), proj)
+        reqs.append((t, w, h, [RB.mesh_draw(mesh, shd, mvp, M.viewport(0, h, w, 0), texture=texr)]))
+    # warm the pools with one of each resolution
+    for k, (w, h) in enumerate(res_list):
+        srv.submit(10 ** 9 + k, reqs[0][3], w, h)
+    got = 0
+    t_warm = time.perf_counter() + 60.0
+    while got < len(res_list) and time.perf_counter() < t_warm:
+        got += len(srv.poll(64, timeout_ms=1000))
+    t0 = time.perf_counter()
+    done = []
+    i = 0
+    deadline = t0 + a.serve_seconds + 60.0
+    while len(done) < n and time.perf_counter() < deadline:
+        now = time.perf_counter() - t0
+        while i < n and reqs[i][0] <= now:
+            _, w, h, draws = reqs[i]
+            srv.submit(i, draws, w, h)
+            i += 1
+        done += srv.poll(64, timeout_ms=0 if i < n else 5)
+        if i < n:
+            nxt = reqs[i][0] - (time.perf_counter() - t0)
+            if nxt > 0.0002:
+                time.sleep(min(nxt, 0.001))
+    t1 = time.perf_counter()
+    srv.close()
+    lat = np.array([(r.done_s - r.submit_s) for r in done]) * 1e3
+    ok = sum(1 for r in done if r.status == 0)
+    if len(done) < n:
+        return {"requests": n, "completed": len(done), "error": "serving run did not finish"}
+    return {"requests": n, "ok": ok, "offered_rate": a.serve_rate, "frames_per_s": round(n / (t1 - t0), 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "resolutions": "uniform over 256^2, 512^2, 1280x720, 1920x1080, 2048^2"}
+
+
+def cpu_baseline(pos, uv, tris, tex, mvps, vp, W, H, seconds):
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    L = O.lib()
+    col = np.zeros((H, W, 4), np.uint8)
+    dep = np.zeros((H, W), np.float32)
+    c = O.OracleCtx(1, 1, 1, 1)
+    clear = np.array([0, 0, 0, 255], np.uint8)
+    st = np.zeros(8, np.uint64)
+    times = []
+    t_end = time.perf_counter() + seconds
+    i = 0
+    tr = np.ascontiguousarray(tris.reshape(-1), np.uint32)
+    ps = np.ascontiguousarray(pos.reshape(-1), np.float32)
+    us = np.ascontiguousarray(uv.reshape(-1), np.float32)
+    tx = np.ascontiguousarray(tex)
+    vpf = np.ascontiguousarray(vp.reshape(-1), np.float32)
+    while (time.perf_counter() < t_end and i < 1000) or i < 3:
+        m = np.ascontiguousarray(mvps[i % len(mvps)].reshape(-1), np.float32)
+        per = C.c_double()
+        s = L.oracle_time_frames(3, O._p(tr, C.c_uint32), tris.shape[0], O._p(ps, C.c_float), O._p(us, C.c_float),
+                                 pos.shape[0], O._p(m, C.c_float), O._p(vpf, C.c_float), None,
+                                 tx.ctypes.data_as(C.POINTER(C.c_uint8)), tex.shape[1], tex.shape[0],
+                                 tex.shape[1], col.ctypes.data_as(C.POINTER(C.c_uint8)), W, H, 0,
+                                 O._p(dep, C.c_float), C.byref(c), clear.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                 float("inf"), 1, C.byref(per), O._p(st, C.c_uint64))
+        assert s >= 0
+        times.append(s)
+        i += 1
+    med = float(np.median(times))
+    return {"value": round(1.0 / med, 3), "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"{len(times)} frames of the same workload (clear + render), median frame time "
+                      f"{med * 1e3:.1f} ms, single thread, C++ restatement of render() (-O2, fp-contract off)"}
+
+
+if __name__ == "__main__":
+    main()

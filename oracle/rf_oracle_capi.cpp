@@ -290,9 +290,9 @@ int oracle_sample(int kind, const uint8_t* tex, uint32_t w, uint32_t h, uint32_t
     return OK;
 }
 
-// Wall-clock seconds of one oracle_render call repeated `reps` times after a
-// fresh clear each time (front/src/lib.rs:103-120 clear semantics: colour
-// filled with `clear_rgba`, depth with 1/depth_clear).
+// Wall-clock seconds of `reps` frames, each a Frame::clear (front/src/lib.rs:
+// 103-120: colour filled with `clear_rgba`, depth with 1/depth_clear) followed
+// by one oracle_render call; both are timed.
 double oracle_time_frames(int kind, const uint32_t* tris, uint64_t n_tris, const float* pos,
                           const float* attr, uint64_t n_verts, const float* mvp,
                           const float* viewport, const uint8_t* flat_rgba, const uint8_t* tex,
@@ -303,6 +303,7 @@ double oracle_time_frames(int kind, const uint32_t* tris, uint64_t n_tris, const
     const uint64_t stride = width * (fmt == PF_RGB8 ? 3 : 4);
     double total = 0.0;
     for (int r = 0; r < reps; ++r) {
+        auto t0 = std::chrono::steady_clock::now();
         // Frame::clear
         for (uint64_t y = 0; y < height; ++y)
             for (uint64_t x = 0; x < width; ++x) {
@@ -311,7 +312,6 @@ double oracle_time_frames(int kind, const uint32_t* tris, uint64_t n_tris, const
                 store_pixel(t, x, y, Rgba{clear_rgba[0], clear_rgba[1], clear_rgba[2], clear_rgba[3]});
                 if (depth) depth[y * width + x] = 1.0f / depth_clear;
             }
-        auto t0 = std::chrono::steady_clock::now();
         int err = oracle_render(kind, tris, n_tris, pos, attr, n_verts, mvp, viewport, flat_rgba,
                                 tex, tex_w, tex_h, tex_stride, color, width, height, stride, fmt,
                                 depth, width, c, stats);

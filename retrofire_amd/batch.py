@@ -1,0 +1,191 @@
+"""Resident meshes/textures/framebuffers, batched frames and the serving loop
+over the C ABI (include/retrofire_amd.h)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from . import Context, Stats, check, default_device
+
+BPP = {0: 4, 1: 3, 2: 4, 3: 4}
+
+
+class Mesh:
+    """A device-resident mesh (geom/mesh.rs:24-30): positions, an optional
+    per-vertex attribute (n_attr floats) and index triples."""
+
+    def __init__(self, pos, tris, attr=None, device=None):
+        self.dev = device or default_device()
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+        n_attr = 0
+        ap = None
+        if attr is not None:
+            attr = np.ascontiguousarray(attr, dtype=np.float32).reshape(pos.shape[0], -1)
+            n_attr = attr.shape[1]
+            ap = attr.ctypes.data_as(C.POINTER(C.c_float))
+        self.n_verts, self.n_tris, self.n_attr = pos.shape[0], tris.shape[0], n_attr
+        self.h = C.c_void_p()
+        check(_ffi.lib().rf_mesh_create(self.dev.h, pos.ctypes.data_as(C.POINTER(C.c_float)), ap, n_attr,
+                                        pos.shape[0], tris.ctypes.data_as(C.POINTER(C.c_uint32)), tris.shape[0],
+                                        C.byref(self.h)))
+
+    def free(self):
+        if self.h:
+            _ffi.lib().rf_mesh_free(self.dev.h, self.h)
+            self.h = C.c_void_p()
+
+
+class Texture:
+    def __init__(self, rgba: np.ndarray, device=None):
+        self.dev = device or default_device()
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        self.h = C.c_void_p()
+        check(_ffi.lib().rf_texture_create(self.dev.h, rgba.ctypes.data_as(C.POINTER(C.c_uint8)), rgba.shape[1],
+                                           rgba.shape[0], rgba.shape[1], C.byref(self.h)))
+
+    def free(self):
+        if self.h:
+            _ffi.lib().rf_texture_free(self.dev.h, self.h)
+            self.h = C.c_void_p()
+
+
+class DeviceFramebuf:
+    """A device-resident colour (+ optional depth) target."""
+
+    def __init__(self, width, height, fmt=0, with_depth=True, device=None):
+        self.dev = device or default_device()
+        self.width, self.height, self.fmt, self.with_depth = width, height, fmt, with_depth
+        self.h = C.c_void_p()
+        check(_ffi.lib().rf_framebuf_create(self.dev.h, width, height, fmt, int(with_depth), C.byref(self.h)))
+
+    def read(self):
+        bpp = BPP[self.fmt]
+        if self.fmt in (2, 3):
+            col = np.empty((self.height, self.width), np.uint32)
+        else:
+            col = np.empty((self.height, self.width, bpp), np.uint8)
+        dep = np.empty((self.height, self.width), np.float32) if self.with_depth else None
+        check(_ffi.lib().rf_framebuf_read(self.dev.h, self.h, col.ctypes.data,
+                                          dep.ctypes.data_as(C.POINTER(C.c_float)) if dep is not None else None))
+        return col, dep
+
+    def free(self):
+        if self.h:
+            _ffi.lib().rf_framebuf_free(self.dev.h, self.h)
+            self.h = C.c_void_p()
+
+
+def mesh_draw(mesh: Mesh, shd, mvp, viewport, first_tri=0, n_tris=0, texture: Texture | None = None):
+    """One render() call over a resident mesh: `shd` is a shader kind object
+    (retrofire_amd.shader.*); textured kinds take a resident `texture`."""
+    d = _ffi.RfMeshDraw()
+    d.mesh = mesh.h.value
+    d.kind = shd.kind
+    m = np.ascontiguousarray(mvp, np.float32).reshape(16)
+    v = np.ascontiguousarray(viewport, np.float32).reshape(16)
+    for i in range(16):
+        d.mvp[i] = m[i]
+        d.viewport[i] = v[i]
+    if shd.kind == 0:
+        for i in range(4):
+            d.flat_color[i] = shd.color[i]
+    if shd.kind >= 2:
+        assert texture is not None
+        d.tex = texture.h.value
+    d.first_tri = first_tri
+    d.n_tris = n_tris
+    return d
+
+
+class Batch:
+    """Frames (target, ctx, draws) planned once and replayed as one HIP graph."""
+
+    def __init__(self, frames, tile_w=0, device=None):
+        self.dev = device or default_device()
+        self._keep = []
+        arr = (_ffi.RfFrame * len(frames))()
+        for i, (fb, ctx, draws) in enumerate(frames):
+            da = (_ffi.RfMeshDraw * max(1, len(draws)))(*draws)
+            self._keep.append(da)
+            arr[i].target = fb.h.value
+            arr[i].ctx = ctx.to_c()
+            arr[i].draws = da
+            arr[i].n_draws = len(draws)
+        self.n = len(frames)
+        self.h = C.c_void_p()
+        check(_ffi.lib().rf_batch_create(self.dev.h, arr, len(frames), tile_w, C.byref(self.h)))
+
+    @property
+    def tile_w(self):
+        return int(_ffi.lib().rf_batch_tile_width(self.h))
+
+    def run(self, sync=True):
+        check(_ffi.lib().rf_batch_run(self.h, int(sync)))
+
+    def wait(self):
+        st = (_ffi.RfStats * self.n)()
+        check(_ffi.lib().rf_batch_wait(self.h, st))
+        out = []
+        for s in st:
+            x = Stats()
+            x.add(s)
+            out.append(x)
+        return out
+
+    def last_ms(self):
+        t, r = C.c_float(), C.c_float()
+        check(_ffi.lib().rf_batch_last_ms(self.h, C.byref(t), C.byref(r)))
+        return t.value
+
+    def profile(self, iters=5):
+        """Mean ms of vertex, setup, scan, bin, binsort, raster (eager run with
+        HIP events on the batch stream)."""
+        ms = (C.c_float * 6)()
+        check(_ffi.lib().rf_batch_profile(self.h, iters, ms))
+        return list(ms)
+
+    def free(self):
+        if self.h:
+            _ffi.lib().rf_batch_free(self.h)
+            self.h = C.c_void_p()
+
+
+class Server:
+    """The serving loop (rf_serve_*): submit requests, poll results."""
+
+    def __init__(self, budget_bytes, max_batch=16, n_streams=2, device=None):
+        self.dev = device or default_device()
+        self.h = C.c_void_p()
+        check(_ffi.lib().rf_serve_open(self.dev.h, budget_bytes, max_batch, n_streams, C.byref(self.h)))
+
+    def submit(self, rid, draws, width, height, fmt=0, with_depth=True, ctx: Context | None = None):
+        ctx = ctx or Context()
+        r = _ffi.RfRequest()
+        r.id = rid
+        da = (_ffi.RfMeshDraw * max(1, len(draws)))(*draws)
+        r.draws = da
+        r.n_draws = len(draws)
+        r.width, r.height, r.fmt, r.with_depth = width, height, fmt, int(with_depth)
+        r.ctx = ctx.to_c()
+        check(_ffi.lib().rf_serve_submit(self.h, C.byref(r)))
+
+    def poll(self, max_results=64, timeout_ms=0, buffers=None):
+        """Returns a list of (rf_result, pixels-or-None). `buffers` maps request
+        id -> preallocated numpy array to receive the pixels."""
+        res = (_ffi.RfResult * max_results)()
+        n = C.c_uint32()
+        ptrs = None
+        if buffers is not None:
+            # pixels are copied in completion order; one buffer per slot
+            ptrs = (C.c_void_p * max_results)(*([b.ctypes.data for b in buffers[:max_results]] +
+                                                [None] * (max_results - len(buffers[:max_results]))))
+        check(_ffi.lib().rf_serve_poll(self.h, res, ptrs, max_results, C.byref(n), timeout_ms))
+        return [res[i] for i in range(n.value)]
+
+    def close(self):
+        if self.h:
+            check(_ffi.lib().rf_serve_close(self.h))
+            self.h = C.c_void_p()

@@ -307,6 +307,7 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     dd.tex_stride = draw->tex_stride;
     dd.flat_rgba = pack_pixel(PIX_RGBA8, draw->flat_color[0], draw->flat_color[1], draw->flat_color[2],
                               draw->flat_color[3]);
+    dd.attr_stride = (uint32_t)na;
     dd.vert_base = 0;
     dd.tri_base = 0;
     std::memcpy(dd.mvp, draw->mvp, 64);

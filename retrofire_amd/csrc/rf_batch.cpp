@@ -1,1 +1,340 @@
+// Batched path: many frames of uneven resolution, each an ordered list of
+// draws over resident meshes, planned once and replayed as one HIP graph.
+//
+// Planning runs the pipeline eagerly to learn the exact clip-fan and bin-run
+// counts (the batch's inputs are fixed, so replays need exactly as much), sizes
+// the scratch, picks the tile width by timing the candidates when asked to,
+// and captures the launch sequence on the batch's own stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/retrofire_amd.h"
 #include "rf_internal.hpp"
+
+namespace rf {
+
+struct Batch {
+    Device* dev = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<rf_frame> frames;
+    std::vector<std::vector<rf_mesh_draw>> draws;
+    std::vector<DrawDesc> h_draws;
+    std::vector<FrameDesc> h_frames;
+    std::vector<uint64_t> frame_prims_i, frame_verts_i, frame_calls;
+    DevBuf d_desc;
+    Scratch sc;
+    BatchDesc B{};
+    uint32_t tile_w = 64;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    FrameStats* h_stats = nullptr;  // pinned
+    uint32_t* h_misc = nullptr;     // pinned
+    bool pending = false;
+};
+
+static rf_status build_descs(Batch& b, uint32_t tile_w) {
+    Device& d = *b.dev;
+    b.h_draws.clear();
+    b.h_frames.clear();
+    b.frame_prims_i.assign(b.frames.size(), 0);
+    b.frame_verts_i.assign(b.frames.size(), 0);
+    b.frame_calls.assign(b.frames.size(), 0);
+    uint64_t vbase = 0, tbase = 0;
+    uint32_t tile_base = 0;
+    for (size_t fi = 0; fi < b.frames.size(); ++fi) {
+        const rf_frame& f = b.frames[fi];
+        const Framebuf* fb = reinterpret_cast<const Framebuf*>(f.target);
+        FrameDesc fd;
+        std::memset(&fd, 0, sizeof(fd));
+        fd.color = (uint8_t*)fb->color.p;
+        fd.depth = fb->has_depth ? (float*)fb->depth.p : nullptr;
+        fd.color_stride = fb->stride;
+        fd.depth_stride = fb->w;
+        fd.width = fb->w;
+        fd.height = fb->h;
+        fd.fmt = fb->fmt;
+        fill_frame_ctx(fd, f.ctx);
+        set_grid(fd, tile_w, tile_base);
+        b.h_frames.push_back(fd);
+        for (const rf_mesh_draw& md : b.draws[fi]) {
+            const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
+            const Tex* t = reinterpret_cast<const Tex*>(md.tex);
+            DrawDesc dd;
+            std::memset(&dd, 0, sizeof(dd));
+            const uint64_t first = md.first_tri;
+            const uint64_t nt = md.n_tris ? md.n_tris : m->n_tris - first;
+            dd.pos = (const float*)m->pos.p;
+            dd.attr = (const float*)m->attr.p;
+            dd.tris = (const uint32_t*)m->tris.p + 3 * first;
+            dd.tex = t ? (const uint8_t*)t->data.p : nullptr;
+            dd.n_verts = (uint32_t)m->n_verts;
+            dd.n_tris = (uint32_t)nt;
+            dd.kind = md.kind;
+            dd.frame = (uint32_t)fi;
+            dd.tex_w = t ? t->w : 0;
+            dd.tex_h = t ? t->h : 0;
+            dd.tex_stride = t ? t->stride : 0;
+            dd.flat_rgba = pack_pixel(PIX_RGBA8, md.flat_color[0], md.flat_color[1], md.flat_color[2],
+                                      md.flat_color[3]);
+            dd.attr_stride = m->n_attr;
+            dd.vert_base = vbase;
+            dd.tri_base = tbase;
+            std::memcpy(dd.mvp, md.mvp, 64);
+            std::memcpy(dd.vp, md.viewport, 64);
+            vbase += m->n_verts;
+            tbase += nt;
+            b.frame_prims_i[fi] += nt;
+            b.frame_verts_i[fi] += m->n_verts;
+            b.frame_calls[fi] += 1;
+            b.h_draws.push_back(dd);
+        }
+    }
+    const size_t db = b.h_draws.size() * sizeof(DrawDesc), fb = b.h_frames.size() * sizeof(FrameDesc);
+    const size_t foff = (db + 255) & ~size_t(255);
+    if (b.d_desc.ensure(foff + fb + 256, &d.bytes)) return RF_E_DEVICE;
+    if (db && hipMemcpy(b.d_desc.p, b.h_draws.data(), db, hipMemcpyHostToDevice)) return RF_E_DEVICE;
+    if (hipMemcpy((uint8_t*)b.d_desc.p + foff, b.h_frames.data(), fb, hipMemcpyHostToDevice)) return RF_E_DEVICE;
+    BatchDesc B;
+    std::memset(&B, 0, sizeof(B));
+    B.draws = (const DrawDesc*)b.d_desc.p;
+    B.frames = (const FrameDesc*)((uint8_t*)b.d_desc.p + foff);
+    B.n_draws = (uint32_t)b.h_draws.size();
+    B.n_frames = (uint32_t)b.h_frames.size();
+    B.n_verts = vbase;
+    B.n_tris = tbase;
+    B.n_groups = (uint32_t)((tbase + WAVE_SIZE - 1) / WAVE_SIZE);
+    B.n_tiles = tile_base;
+    B.tile_w = tile_w;
+    B.tile_h = TILE_ROWS;
+    B.extra_cap = std::max<uint32_t>(b.B.extra_cap, 1024u);
+    B.runs_cap = std::max<uint32_t>(b.B.runs_cap, 4 * B.n_groups + B.n_tiles + 1024);
+    b.B = B;
+    b.tile_w = tile_w;
+    return RF_OK;
+}
+
+// Eager run that sizes the scratch exactly for this batch (with headroom).
+static rf_status size_batch(Batch& b) {
+    std::vector<FrameStats> fs;
+    uint32_t err = 0;
+    rf_status st = run_sized(*b.dev, b.sc, b.B, b.stream, {}, fs, &err, nullptr);
+    if (st) return st;
+    if ((st = map_err(err))) return st;
+    return RF_OK;
+}
+
+static float time_eager(Batch& b, int reps) {
+    if (hipEventRecord(b.ev0, b.stream)) return -1.0f;
+    for (int i = 0; i < reps; ++i)
+        if (launch_pipeline(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles))) return -1.0f;
+    if (hipEventRecord(b.ev1, b.stream) || hipEventSynchronize(b.ev1)) return -1.0f;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b.ev0, b.ev1)) return -1.0f;
+    return ms / reps;
+}
+
+static void destroy_graph(Batch& b) {
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b.exec = nullptr;
+    b.graph = nullptr;
+}
+
+static rf_status capture(Batch& b) {
+    destroy_graph(b);
+    if (hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal)) return RF_E_DEVICE;
+    const hipError_t le = launch_pipeline(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles));
+    const hipError_t ee = hipStreamEndCapture(b.stream, &b.graph);
+    if (le || ee || !b.graph) return RF_E_DEVICE;
+    if (hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0)) return RF_E_DEVICE;
+    return RF_OK;
+}
+
+static void batch_release(Batch* b) {
+    if (!b) return;
+    Device& d = *b->dev;
+    (void)hipSetDevice(d.ordinal);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    destroy_graph(*b);
+    b->sc.release(&d.bytes);
+    b->d_desc.release(&d.bytes);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->h_stats) (void)hipHostFree(b->h_stats);
+    if (b->h_misc) (void)hipHostFree(b->h_misc);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+}  // namespace rf
+
+using namespace rf;
+
+extern "C" {
+
+rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_frames, uint32_t tile_w,
+                          rf_batch** out) {
+    if (!dev || !out || !frames || n_frames == 0) return RF_E_ARG;
+    if (tile_w != 0 && tile_w != 32 && tile_w != 64) return RF_E_ARG;
+    Device& d = *reinterpret_cast<Device*>(dev);
+    for (uint32_t i = 0; i < n_frames; ++i) {
+        const rf_frame& f = frames[i];
+        if (!f.target || (f.n_draws && !f.draws)) return RF_E_ARG;
+        rf_status st = check_ctx(&f.ctx);
+        if (st) return st;
+        for (uint32_t k = 0; k < f.n_draws; ++k) {
+            const rf_mesh_draw& md = f.draws[k];
+            if (!md.mesh || md.kind > KIND_TEX_REPEAT_POT) return RF_E_ARG;
+            const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
+            const Tex* t = reinterpret_cast<const Tex*>(md.tex);
+            if ((int)m->n_attr < kind_attrs(md.kind)) return RF_E_ARG;
+            if (md.first_tri > m->n_tris || md.first_tri + md.n_tris > m->n_tris) return RF_E_ARG;
+            if ((st = check_tex(md.kind, t ? t->data.p : nullptr, t ? t->w : 0, t ? t->h : 0, t ? t->stride : 0)))
+                return st;
+        }
+    }
+    if (hipSetDevice(d.ordinal)) return RF_E_DEVICE;
+    Batch* b = new (std::nothrow) Batch();
+    if (!b) return RF_E_DEVICE;
+    b->dev = &d;
+    b->frames.assign(frames, frames + n_frames);
+    b->draws.resize(n_frames);
+    for (uint32_t i = 0; i < n_frames; ++i) {
+        b->draws[i].assign(frames[i].draws, frames[i].draws + frames[i].n_draws);
+        b->frames[i].draws = nullptr;
+    }
+    rf_status st = RF_OK;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) || hipEventCreate(&b->ev0) ||
+        hipEventCreate(&b->ev1) || hipHostMalloc((void**)&b->h_stats, n_frames * sizeof(FrameStats), 0) ||
+        hipHostMalloc((void**)&b->h_misc, 64, 0)) {
+        batch_release(b);
+        return RF_E_DEVICE;
+    }
+    // Tile width: forced, or the faster of the candidates on this batch.
+    std::vector<uint32_t> cands = tile_w ? std::vector<uint32_t>{tile_w} : std::vector<uint32_t>{32, 64};
+    float best = 1e30f;
+    uint32_t best_w = cands[0];
+    for (uint32_t w : cands) {
+        if ((st = build_descs(*b, w)) || (st = size_batch(*b))) {
+            batch_release(b);
+            return st;
+        }
+        if (cands.size() > 1) {
+            const float ms = time_eager(*b, 3);
+            if (ms < 0) {
+                batch_release(b);
+                return RF_E_DEVICE;
+            }
+            if (ms < best) {
+                best = ms;
+                best_w = w;
+            }
+        }
+    }
+    if (best_w != b->tile_w && ((st = build_descs(*b, best_w)) || (st = size_batch(*b)))) {
+        batch_release(b);
+        return st;
+    }
+    // headroom, then the final scratch and the graph
+    b->B.extra_cap += b->B.extra_cap / 8 + 64;
+    b->B.runs_cap += b->B.runs_cap / 8 + 64;
+    if (b->sc.prepare(b->B, &d.bytes) || (st = capture(*b))) {
+        batch_release(b);
+        return st ? st : RF_E_DEVICE;
+    }
+    *out = reinterpret_cast<rf_batch*>(b);
+    return RF_OK;
+}
+
+rf_status rf_batch_run(rf_batch* bh, int sync) {
+    if (!bh) return RF_E_ARG;
+    Batch& b = *reinterpret_cast<Batch*>(bh);
+    if (hipSetDevice(b.dev->ordinal)) return RF_E_DEVICE;
+    if (hipEventRecord(b.ev0, b.stream) || hipGraphLaunch(b.exec, b.stream) || hipEventRecord(b.ev1, b.stream))
+        return RF_E_DEVICE;
+    if (hipMemcpyAsync(b.h_stats, b.sc.stats.p, b.B.n_frames * sizeof(FrameStats), hipMemcpyDeviceToHost,
+                       b.stream) ||
+        hipMemcpyAsync(b.h_misc, b.sc.misc.p, 16, hipMemcpyDeviceToHost, b.stream))
+        return RF_E_DEVICE;
+    b.pending = true;
+    if (sync) return rf_batch_wait(bh, nullptr);
+    return RF_OK;
+}
+
+rf_status rf_batch_wait(rf_batch* bh, rf_stats* stats) {
+    if (!bh) return RF_E_ARG;
+    Batch& b = *reinterpret_cast<Batch*>(bh);
+    if (hipStreamSynchronize(b.stream)) return RF_E_DEVICE;
+    b.pending = false;
+    rf_status st = map_err(b.h_misc[2]);
+    if (st) return st;
+    if (stats) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, b.ev0, b.ev1);
+        for (uint32_t i = 0; i < b.B.n_frames; ++i) {
+            rf_stats& s = stats[i];
+            s.calls = b.frame_calls[i];
+            s.prims_i = b.frame_prims_i[i];
+            s.verts_i = b.frame_verts_i[i];
+            s.prims_o = b.h_stats[i].prims_o;
+            s.verts_o = b.h_stats[i].verts_o;
+            s.frags_i = b.h_stats[i].frags_i;
+            s.frags_o = b.h_stats[i].frags_o;
+            s.seconds = ms * 1e-3;
+        }
+    }
+    return RF_OK;
+}
+
+rf_status rf_batch_free(rf_batch* bh) {
+    if (!bh) return RF_E_ARG;
+    batch_release(reinterpret_cast<Batch*>(bh));
+    return RF_OK;
+}
+
+uint32_t rf_batch_tile_width(const rf_batch* bh) {
+    return bh ? reinterpret_cast<const Batch*>(bh)->tile_w : 0;
+}
+
+rf_status rf_batch_last_ms(const rf_batch* bh, float* total_ms, float* raster_ms) {
+    if (!bh) return RF_E_ARG;
+    const Batch& b = *reinterpret_cast<const Batch*>(bh);
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b.ev0, b.ev1)) return RF_E_DEVICE;
+    if (total_ms) *total_ms = ms;
+    if (raster_ms) *raster_ms = -1.0f;  // not separable inside the graph; see rf_batch_profile
+    return RF_OK;
+}
+
+rf_status rf_batch_profile(rf_batch* bh, int iters, float* ms) {
+    if (!bh || !ms || iters <= 0) return RF_E_ARG;
+    Batch& b = *reinterpret_cast<Batch*>(bh);
+    if (hipSetDevice(b.dev->ordinal)) return RF_E_DEVICE;
+    hipEvent_t ev[7];
+    for (auto& e : ev)
+        if (hipEventCreate(&e)) return RF_E_DEVICE;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    rf_status st = RF_OK;
+    for (int it = 0; it < iters && !st; ++it) {
+        if (launch_pipeline_timed(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles), ev) ||
+            hipEventSynchronize(ev[6])) {
+            st = RF_E_DEVICE;
+            break;
+        }
+        for (int k = 0; k < 6; ++k) {
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+            acc[k] += t;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 6; ++k) ms[k] = (float)(acc[k] / iters);
+    return st;
+}
+
+}  // extern "C"

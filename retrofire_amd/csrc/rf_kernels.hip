@@ -94,12 +94,13 @@ __global__ __launch_bounds__(256) void k_vertex(BatchDesc B) {
     const float z = dot4(m + 8, px, py, pz, 1.0f);
     const float w = dot4(m + 12, px, py, pz, 1.0f);
     const int na = kind_attrs(D->kind);
+    const uint64_t as = D->attr_stride;
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
     if (na >= 2) {
-        a0 = D->attr[na * i + 0];
-        a1 = D->attr[na * i + 1];
+        a0 = D->attr[as * i + 0];
+        a1 = D->attr[as * i + 1];
     }
-    if (na == 3) a2 = D->attr[na * i + 2];
+    if (na == 3) a2 = D->attr[as * i + 2];
     const uint32_t oc = outcode(x, y, z, w);
     B.cv[slot] = make_float4(x, y, z, w);
     B.ca[slot] = make_float4(a0, a1, a2, u2f(oc));

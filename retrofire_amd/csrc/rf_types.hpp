@@ -59,6 +59,8 @@ struct DrawDesc {
     uint32_t kind, frame;
     uint32_t tex_w, tex_h, tex_stride;  // stride in texels
     uint32_t flat_rgba;                 // r | g << 8 | b << 16 | a << 24
+    uint32_t attr_stride;               // floats per vertex in `attr`
+    uint32_t pad0;
     uint64_t vert_base;                 // first slot in the batch vertex buffers
     uint64_t tri_base;                  // first global triangle index
     float mvp[16];                      // row-major ProjMat3 (the uniform)
