@@ -26,33 +26,32 @@ FLAGS = [
 ]
 
 
-def _objs():
-    return [os.path.join(CSRC, "_build", os.path.splitext(s)[0] + ".o") for s in SOURCES]
-
-
-def build(verbose=False, jobs=4):
-    os.makedirs(os.path.join(CSRC, "_build"), exist_ok=True)
+def build(verbose=False, jobs=4, defines=(), out=None, build_dir=None):
+    bdir = build_dir or os.path.join(CSRC, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    objs = [os.path.join(bdir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
     procs = []
-    for src, obj in zip(SOURCES, _objs()):
-        cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    for src, obj in zip(SOURCES, objs):
+        cmd = [HIPCC] + FLAGS + [f"-D{d}" for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
     failed = False
     for src, p in procs:
-        out, _ = p.communicate()
+        log, _ = p.communicate()
         if p.returncode != 0:
             failed = True
-            sys.stderr.write(f"--- {src}\n{out.decode(errors='replace')}\n")
-        elif verbose and out:
-            sys.stderr.write(out.decode(errors="replace"))
+            sys.stderr.write(f"--- {src}\n{log.decode(errors='replace')}\n")
+        elif verbose and log:
+            sys.stderr.write(log.decode(errors="replace"))
     if failed:
         raise RuntimeError("hipcc failed")
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + _objs() + ["-lpthread"]
+    lib = out or LIB
+    tmp = lib + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -15,7 +15,10 @@ namespace rf {
 
 constexpr uint32_t WAVE_SIZE = 64;
 constexpr uint32_t TILE_ROWS = 64;          // rasterizer tile height (one lane per row)
-constexpr uint32_t RASTER_WG_PER_CU = 4;    // persistent rasterizer workgroups per CU
+#ifndef RF_RASTER_WG_PER_CU
+#define RF_RASTER_WG_PER_CU 4
+#endif
+constexpr uint32_t RASTER_WG_PER_CU = RF_RASTER_WG_PER_CU;  // persistent rasterizer workgroups per CU
 
 // Grow-only device allocation.
 struct DevBuf {

@@ -23,6 +23,7 @@
 // varying stepping per pixel) are reproduced as serial sums; no closed forms.
 #include <hip/hip_runtime.h>
 
+#include "rf_ff.hpp"
 #include "rf_math.hpp"
 #include "rf_types.hpp"
 
@@ -31,6 +32,15 @@ namespace rf {
 constexpr int WAVE = 64;
 constexpr int TILE_H = 64;  // one lane per tile row
 constexpr int BIN_MARGIN = 2;  // extra pixels around a triangle's bounding box
+// Tuning constants (overridable at build time for sweeps).
+#ifndef RF_FF_MIN
+#define RF_FF_MIN 12
+#endif
+#ifndef RF_CHUNK
+#define RF_CHUNK 8
+#endif
+constexpr int FF_MIN = RF_FF_MIN;  // step counts above this use the exact fast-forward
+constexpr int CHUNK = RF_CHUNK;    // pixels per lane processed together in a span
 
 // ---------------------------------------------------------------------------
 // wave helpers
@@ -840,17 +850,28 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         const int32_t k = row - (int32_t)T.y0i;
                         if (k < 0 || (uint32_t)k >= T.n) continue;
                         // ScanlineIter::next: left/right iterators stepped k times
-                        float lx = T.lx, lz = T.lz, l0 = T.la[0], l1 = T.la[1], l2 = T.la[2];
-                        float rx = T.rx;
-                        for (int32_t q = 0; q < k; ++q) {
-                            lx = lx + T.dlx;
-                            lz = lz + T.dlz;
-                            if (na > 0) {
-                                l0 = l0 + T.dla[0];
-                                l1 = l1 + T.dla[1];
-                                if (na > 2) l2 = l2 + T.dla[2];
+                        // (exactly; long runs through the binade fast-forward)
+                        float lx, lz, l0, l1, l2, rx;
+                        if (k <= FF_MIN) {
+                            lx = T.lx; lz = T.lz; l0 = T.la[0]; l1 = T.la[1]; l2 = T.la[2];
+                            rx = T.rx;
+                            for (int32_t q = 0; q < k; ++q) {
+                                lx = lx + T.dlx;
+                                lz = lz + T.dlz;
+                                if (na > 0) {
+                                    l0 = l0 + T.dla[0];
+                                    l1 = l1 + T.dla[1];
+                                    if (na > 2) l2 = l2 + T.dla[2];
+                                }
+                                rx = rx + T.drx;
                             }
-                            rx = rx + T.drx;
+                        } else {
+                            lx = ff_add(T.lx, T.dlx, (uint32_t)k);
+                            lz = ff_add(T.lz, T.dlz, (uint32_t)k);
+                            l0 = na > 0 ? ff_add(T.la[0], T.dla[0], (uint32_t)k) : T.la[0];
+                            l1 = na > 0 ? ff_add(T.la[1], T.dla[1], (uint32_t)k) : T.la[1];
+                            l2 = na > 2 ? ff_add(T.la[2], T.dla[2], (uint32_t)k) : T.la[2];
+                            rx = ff_add(T.rx, T.drx, (uint32_t)k);
                         }
                         const float rx0 = round_up_to_half(lx), rx1 = round_up_to_half(rx);
                         // v0.lerp(&v0.step(&dv_dx), x0 - v0.x)
@@ -879,40 +900,62 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         int64_t j1 = (int64_t)x0 + TW - (int64_t)xs;
                         if (j1 > (int64_t)nfrag) j1 = (int64_t)nfrag;
                         if (j0 >= j1) continue;
-                        for (int64_t q = 0; q < j0; ++q) {
-                            z = z + T.dvz;
+                        // the fragment iterator stepped j0 times (Scanline::fragments)
+                        if (j0 <= FF_MIN) {
+                            for (int64_t q = 0; q < j0; ++q) {
+                                z = z + T.dvz;
+                                if (na > 0) {
+                                    c0 = c0 + T.dva[0];
+                                    c1 = c1 + T.dva[1];
+                                    if (na > 2) c2 = c2 + T.dva[2];
+                                }
+                            }
+                        } else {
+                            z = ff_add(z, T.dvz, (uint32_t)j0);
                             if (na > 0) {
-                                c0 = c0 + T.dva[0];
-                                c1 = c1 + T.dva[1];
-                                if (na > 2) c2 = c2 + T.dva[2];
+                                c0 = ff_add(c0, T.dva[0], (uint32_t)j0);
+                                c1 = ff_add(c1, T.dva[1], (uint32_t)j0);
+                                if (na > 2) c2 = ff_add(c2, T.dva[2], (uint32_t)j0);
                             }
                         }
                         const uint32_t lrow = lane;
-                        for (int64_t q = j0; q < j1; ++q) {
-                            const uint32_t col = (uint32_t)((int64_t)xs + q - (int64_t)x0);
-                            const uint32_t li = col * LD + lrow;
-                            if (has_depth) {
-                                const float cur = s_dep[li];
-                                if (depth_pass(F.depth_test, z, cur)) {
-                                    const uint32_t px = shade(S, F.fmt, z, c0, c1, c2);
-                                    if (F.color_write) {
-                                        ++fo_acc;
-                                        s_col[li] = px;
-                                    }
-                                    if (F.depth_write) s_dep[li] = z;
-                                }
-                            } else {
-                                const uint32_t px = shade(S, F.fmt, z, c0, c1, c2);
-                                if (F.color_write) {
-                                    ++fo_acc;
-                                    s_col[li] = px;
+                        const uint32_t col0 = (uint32_t)((int64_t)xs + j0 - (int64_t)x0);
+                        const uint32_t npx = (uint32_t)(j1 - j0);
+                        // pixels in chunks of CHUNK: step the varyings, test depth,
+                        // then fetch all texels of the chunk before writing any
+                        for (uint32_t q0 = 0; q0 < npx; q0 += CHUNK) {
+                            float zz[CHUNK], v0[CHUNK], v1[CHUNK], v2[CHUNK];
+                            bool pass[CHUNK];
+#pragma unroll
+                            for (int p = 0; p < CHUNK; ++p) {
+                                zz[p] = z;
+                                v0[p] = c0;
+                                v1[p] = c1;
+                                v2[p] = c2;
+                                z = z + T.dvz;
+                                if (na > 0) {
+                                    c0 = c0 + T.dva[0];
+                                    c1 = c1 + T.dva[1];
+                                    if (na > 2) c2 = c2 + T.dva[2];
                                 }
                             }
-                            z = z + T.dvz;
-                            if (na > 0) {
-                                c0 = c0 + T.dva[0];
-                                c1 = c1 + T.dva[1];
-                                if (na > 2) c2 = c2 + T.dva[2];
+                            uint32_t px[CHUNK];
+#pragma unroll
+                            for (int p = 0; p < CHUNK; ++p) {
+                                const bool in = q0 + p < npx;
+                                const uint32_t li = (col0 + q0 + p) * LD + lrow;
+                                pass[p] = in && (!has_depth || depth_pass(F.depth_test, zz[p], s_dep[in ? li : lrow]));
+                                px[p] = pass[p] ? shade(S, F.fmt, zz[p], v0[p], v1[p], v2[p]) : 0u;
+                            }
+#pragma unroll
+                            for (int p = 0; p < CHUNK; ++p) {
+                                if (!pass[p]) continue;
+                                const uint32_t li = (col0 + q0 + p) * LD + lrow;
+                                if (F.color_write) {
+                                    ++fo_acc;
+                                    s_col[li] = px[p];
+                                }
+                                if (has_depth && F.depth_write) s_dep[li] = zz[p];
                             }
                         }
                     }

@@ -156,3 +156,40 @@ def test_errors():
         rf.render(np.array([[0, 1, 2]]), verts, rf.shader.Flat(), M.identity(), M.viewport(0, 0, 16, 16),
                   rf.Colorbuf(fb), rf.Context(face_cull=None))
     assert e.value.status == -2
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_huge_triangles_long_spans(kind):
+    """Full-screen perspective geometry at 2048x2048: rows thousands of steps
+    below a triangle's top and spans thousands of pixels long, so every tile
+    enters its spans through the exact fast-forward of the serial sums."""
+    rng = np.random.default_rng(40 + kind)
+    w = h = 2048
+    n = 24
+    pos = np.concatenate([rng.uniform([-30, -30, -2], [30, 30, 40], size=(3 * n, 3))]).astype(np.float32)
+    tris = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    attr = rng.uniform(0, 1, size=(3 * n, 3)).astype(np.float32) if kind == 1 else \
+        rng.uniform(-3, 4, size=(3 * n, 2)).astype(np.float32)
+    tex = rng.integers(0, 256, size=(64, 128, 4), dtype=np.uint8) if kind >= 2 else None
+    mvp = M.then(M.translate3(0, 0, 1.0), M.perspective(0.7, 1.0, 0.1, 100.0))
+    vp = M.viewport(0, h, w, 0)
+    ctx = rf.Context(clear=True, face_cull=None)
+    o, g = run_both(kind, tris, pos, attr, mvp, vp, w, h, tex=tex, ctx=ctx)
+    assert_same(o, g)
+    assert g[2][6] > w * h // 4
+
+
+def test_tall_thin_fan():
+    """Slivers spanning the full height (row counts >> 64) with tiny widths."""
+    w, h = 300, 2000
+    n = 60
+    xs = np.linspace(-1, 1, n + 1).astype(np.float32)
+    pos = []
+    for i in range(n):
+        pos += [[xs[i], -1.2, 0.5], [xs[i + 1], -1.2, 0.5], [(xs[i] + xs[i + 1]) / 2, 1.2, 0.3]]
+    pos = np.array(pos, np.float32)
+    cols = np.random.default_rng(3).uniform(0, 1, size=(pos.shape[0], 3)).astype(np.float32)
+    tris = np.arange(pos.shape[0], dtype=np.uint32).reshape(-1, 3)
+    mvp = M.then(M.translate3(0, 0, 2.0), M.perspective(1.0, np.float32(w) / np.float32(h), 0.1, 10.0))
+    o, g = run_both(1, tris, pos, cols, mvp, M.viewport(0, h, w, 0), w, h, ctx=rf.Context(clear=True, face_cull=None))
+    assert_same(o, g)
