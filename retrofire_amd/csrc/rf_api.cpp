@@ -74,6 +74,7 @@ rf_status check_tex(uint32_t kind, const void* tex, uint32_t w, uint32_t h, uint
     if (kind == KIND_FLAT || kind == KIND_GOURAUD) return RF_OK;
     if (kind > KIND_TEX_REPEAT_POT) return RF_E_UNSUPPORTED;
     if (!tex || w == 0 || h == 0 || stride < w) return RF_E_ARG;
+    if ((uint64_t)stride * h > 0xFFFFFFFFull) return RF_E_ARG;  // texel indices are 32-bit
     if (kind == KIND_TEX_REPEAT_POT && ((w & (w - 1)) || (h & (h - 1)))) return RF_E_ARG;
     return RF_OK;
 }
@@ -256,7 +257,8 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     if (st) return st;
     if (draw->kind > KIND_TEX_REPEAT_POT) return RF_E_UNSUPPORTED;
     if ((st = check_tex(draw->kind, draw->tex, draw->tex_w, draw->tex_h, draw->tex_stride))) return st;
-    if (!target->color || target->width == 0 || target->height == 0 || target->fmt > PIX_U32_XRGB)
+    if (!target->color || target->width == 0 || target->height == 0 || target->fmt > PIX_U32_XRGB ||
+        target->width > MAX_DIM || target->height > MAX_DIM)
         return RF_E_ARG;
     const uint32_t bpp = bytes_per_pixel(target->fmt);
     if (target->color_stride < target->width * bpp) return RF_E_ARG;
@@ -449,7 +451,7 @@ rf_status rf_texture_free(rf_device* dev, rf_texture* tex) {
 
 rf_status rf_framebuf_create(rf_device* dev, uint32_t w, uint32_t h, uint32_t fmt, int with_depth,
                              rf_framebuf** out) {
-    if (!dev || !out || !w || !h || fmt > PIX_U32_XRGB) return RF_E_ARG;
+    if (!dev || !out || !w || !h || fmt > PIX_U32_XRGB || w > MAX_DIM || h > MAX_DIM) return RF_E_ARG;
     Device& d = *reinterpret_cast<Device*>(dev);
     if (hipSetDevice(d.ordinal)) return RF_E_DEVICE;
     Framebuf* f = new (std::nothrow) Framebuf();

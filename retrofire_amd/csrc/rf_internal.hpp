@@ -14,6 +14,7 @@
 namespace rf {
 
 constexpr uint32_t WAVE_SIZE = 64;
+constexpr uint32_t MAX_DIM = 16384;         // largest target side (span records pack x in 16 bits)
 constexpr uint32_t TILE_ROWS = 64;          // rasterizer tile height (one lane per row)
 #ifndef RF_RASTER_WG_PER_CU
 #define RF_RASTER_WG_PER_CU 4

@@ -632,27 +632,40 @@ __device__ __forceinline__ void scan_setup(Trap& T, float y0, float y1, const fl
     T.y0i = y0r - 0.5f;
 }
 
-// tri_fill (raster.rs:185-224) setup for one screen triangle.
-__device__ __forceinline__ void tri_setup(Slot& S, const float* a, const float* b, const float* c,
-                                          int na) {
-    const float* v[3] = {a, b, c};
-    // stable sort by total_cmp of y
-    if (total_key(v[1][1]) < total_key(v[0][1])) { const float* t = v[0]; v[0] = v[1]; v[1] = t; }
-    if (total_key(v[2][1]) < total_key(v[1][1])) { const float* t = v[1]; v[1] = v[2]; v[2] = t; }
-    if (total_key(v[1][1]) < total_key(v[0][1])) { const float* t = v[0]; v[0] = v[1]; v[1] = t; }
-    const float* top = v[0];
-    const float* mid0 = v[1];
-    const float* bot = v[2];
+// tri_fill (raster.rs:185-224) setup for one screen triangle (vertices as
+// x, y, z, a0, a1, a2; sorted in registers).
+__device__ __forceinline__ void swap6(float* a, float* b) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const float t = a[k];
+        a[k] = b[k];
+        b[k] = t;
+    }
+}
+__device__ __forceinline__ void tri_setup(Slot& S, float (&va)[6], float (&vb)[6], float (&vc)[6], int na) {
+    // stable sort by total_cmp of y (slice::sort_by is stable)
+    if (total_key(vb[1]) < total_key(va[1])) swap6(va, vb);
+    if (total_key(vc[1]) < total_key(vb[1])) swap6(vb, vc);
+    if (total_key(vb[1]) < total_key(va[1])) swap6(va, vb);
+    const float* top = va;
+    const float* mid0 = vb;
+    const float* bot = vc;
     const float top_y = top[1], mid_y = mid0[1], bot_y = bot[1];
     const float tt = (mid_y - top_y) / (bot_y - top_y);
     float mid1[6];
+#pragma unroll
     for (int k = 0; k < 6; ++k) mid1[k] = top[k] + (bot[k] - top[k]) * tt;
     const bool l_is_mid0 = mid0[0] < mid1[0];
-    const float* left = l_is_mid0 ? mid0 : mid1;
-    const float* right = l_is_mid0 ? mid1 : mid0;
+    float left[6], right[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        left[k] = l_is_mid0 ? mid0[k] : mid1[k];
+        right[k] = l_is_mid0 ? mid1[k] : mid0[k];
+    }
     scan_setup(S.t[0], top_y, mid_y, top, left, top, right, na);
     scan_setup(S.t[1], mid_y, bot_y, left, bot, right, bot, na);
     int32_t lo = 0x7FFFFFFF, hi = -0x7FFFFFFF;
+#pragma unroll
     for (int h = 0; h < 2; ++h) {
         if (S.t[h].n == 0) continue;
         const int32_t y0 = (int32_t)S.t[h].y0i;
@@ -664,8 +677,15 @@ __device__ __forceinline__ void tri_setup(Slot& S, const float* a, const float* 
     S.yhi = hi;
 }
 
+// What the fragment stage needs of a triangle's draw.
+struct ShadeInfo {
+    uint32_t kind, flat;
+    const uint8_t* tex;
+    uint32_t tw, th, tstride, pad;
+};
+
 // Built-in fragment shader for one fragment, result in the target format.
-__device__ __forceinline__ uint32_t shade(const Slot& S, uint32_t fmt, float z, float a0, float a1,
+__device__ __forceinline__ uint32_t shade(const ShadeInfo& S, uint32_t fmt, float z, float a0, float a1,
                                           float a2) {
     uint32_t r, g, b, a;
     switch (S.kind) {
@@ -739,14 +759,100 @@ __device__ __forceinline__ uint32_t find_frame_tile(const FrameDesc* f, uint32_t
     return lo;
 }
 
+// Per-triangle data the row lanes read during the pixel phase.
+struct SlotP {
+    float dv[2][4];        // d(z, a0, a1, a2)/dx of the upper and lower trapezoid
+    uint32_t kind;
+    uint32_t p0, p1, p2;   // flat: packed colour; textured: width, height, row stride (texels)
+    uint32_t draw;
+    uint32_t rec_base;     // first span record
+    int32_t ra, rb;        // tile rows [ra, rb) with a record
+};
+
+// Span record of one (triangle, row): the varyings at the span's first pixel
+// (after the x0 adjustment of ScanlineIter::next), its pixel range and
+// fragment count.
+struct SpanRec {
+    float z, a0, a1, a2;
+    uint32_t xs_nfrag_h;   // xs | fragment count << 15 | trapezoid << 31
+};
+
+#ifndef RF_REC_CAP
+#define RF_REC_CAP 384
+#endif
+constexpr int REC_CAP = RF_REC_CAP;  // span records per batch
+constexpr int FQ_CAP = 512;     // fan queue (ring); one run adds at most 64 * 7 fans
+constexpr int KMAX = 160;       // vertical steps above which the fast-forward is used
+constexpr int JMAX = 200;       // horizontal steps above which the fast-forward is used
+
+// Tile rasterizer. One wave owns one tile (TILE_H rows x TW columns) at a
+// time; the tile's depth and a per-pixel record of its last passing fragment
+// live in LDS. Triangles are consumed in submission order in batches of up to
+// 64:
+//   setup  lane = triangle: tri_fill/scan setup (raster.rs:185-302);
+//   edges  lane = triangle: walks its rows inside the tile with the serial
+//          edge recurrence (ScanlineIter::next, raster.rs:84-113), entering
+//          the tile through the exact fast-forward when it starts far above,
+//          and writes one span record per row;
+//   spans  lane = row: walks the span records of its row in triangle order,
+//          stepping the varyings pixel by pixel (Scanline::fragments,
+//          raster.rs:61-68) and applying the depth test and depth write of
+//          rasterize_fb (target.rs:163-198). Each pixel row belongs to one
+//          lane, so every pixel sees its fragments in submission order.
+// A passing fragment that writes colour leaves in the pixel's record what its
+// colour is made of: the colour itself for the flat and Gouraud shaders, the
+// draw and the sampled texel's index for the textured ones (the samplers'
+// coordinate arithmetic runs in the span walk; only the texel fetch waits).
+// The colour a pixel ends with is that of its last passing fragment, so the
+// fetches run once per written pixel at the end of the tile, all pixels in
+// parallel, and colour and depth go straight to the target.
+constexpr uint32_t TAG_NONE = 0xFFFFFFFFu;   // no fragment wrote colour
+constexpr uint32_t TAG_COLOR = 0xFFFFFFFEu;  // payload is the colour; else tag = draw, payload = texel
+
+__device__ __forceinline__ void frag_payload(const SlotP& P, uint32_t fmt, float z, float a0, float a1,
+                                             float a2, uint32_t& tag, uint32_t& pay) {
+    switch (P.kind) {
+        case KIND_FLAT:
+            tag = TAG_COLOR;
+            pay = P.p0;
+            return;
+        case KIND_GOURAUD:  // Color3f::to_color4 (color.rs:253-263)
+            tag = TAG_COLOR;
+            pay = pack_pixel(fmt, as_u8(256.0f * a0), as_u8(256.0f * a1), as_u8(256.0f * a2), 0xFF);
+            return;
+        default: {
+            // Scanline::fragments z_div (raster.rs:61-68, vec.rs:666-676)
+            const float u = a0 / z, v = a1 / z;
+            const float tw = (float)P.p0, th = (float)P.p1;
+            uint32_t ui, vi;
+            if (P.kind == KIND_TEX_CLAMP) {  // SamplerClamp (tex.rs:280-303)
+                float su = u * tw, sv = v * th;
+                const float mu = tw - 1.0f, mv = th - 1.0f;
+                if (su < 0.0f) su = 0.0f;
+                if (su > mu) su = mu;
+                if (sv < 0.0f) sv = 0.0f;
+                if (sv > mv) sv = mv;
+                ui = as_u32(floorf(su));
+                vi = as_u32(floorf(sv));
+            } else {  // SamplerRepeatPot (tex.rs:240-266)
+                ui = (uint32_t)as_i32(floorf(tw * u)) & (P.p0 - 1);
+                vi = (uint32_t)as_i32(floorf(th * v)) & (P.p1 - 1);
+            }
+            tag = P.draw;
+            pay = vi * P.p2 + ui;
+        }
+    }
+}
+
 template <int TW>
 __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
     constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
-    __shared__ uint32_t s_col[TW * LD];
-    __shared__ float s_dep[TW * LD];
-    __shared__ Slot s_slot[WAVE];
-    __shared__ unsigned long long s_rowmask[TILE_H];
-    __shared__ uint16_t s_fan[WAVE * 8];
+    constexpr int NPX = TW * LD;
+    __shared__ float s_dep[NPX];
+    __shared__ uint32_t s_tag[NPX], s_pay[NPX];  // the pixel's colour source
+    __shared__ SlotP s_slot[WAVE];
+    __shared__ SpanRec s_rec[REC_CAP];
+    __shared__ uint32_t s_fq[FQ_CAP];   // g << 3 | fan
     __shared__ uint32_t s_tile;
 
     const uint32_t lane = lane_id();
@@ -768,208 +874,277 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
         const bool has_depth = F.depth != nullptr;
         const uint32_t ncols = min((uint32_t)TW, F.width - x0);
         const uint32_t nrows = min((uint32_t)TILE_H, F.height - y0);
+        const uint32_t npx = ncols * nrows;
 
         if (nruns == 0) {
             // Nothing drawn here: the fused clear is the only write.
-            if (F.clear_color)
-                for (uint32_t r = 0; r < nrows; ++r)
-                    for (uint32_t c = lane; c < ncols; c += WAVE) store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
-            if (has_depth && F.clear_depth)
-                for (uint32_t r = 0; r < nrows; ++r)
-                    for (uint32_t c = lane; c < ncols; c += WAVE)
-                        F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = F.clear_depth_value;
+            for (uint32_t i = lane; i < npx; i += WAVE) {
+                const uint32_t r = i / ncols, c = i - r * ncols;
+                if (F.clear_color) store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
+                if (has_depth && F.clear_depth)
+                    F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = F.clear_depth_value;
+            }
             continue;
         }
 
-        // tile init: fused clear or load from the target
-        for (uint32_t r = 0; r < nrows; ++r)
-            for (uint32_t c = lane; c < ncols; c += WAVE) {
-                s_col[c * LD + r] = F.clear_color ? F.clear_pixel : load_pixel(F, x0 + c, y0 + r);
-                if (has_depth)
-                    s_dep[c * LD + r] = F.clear_depth ? F.clear_depth_value
-                                                      : F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c];
-            }
-        s_rowmask[lane] = 0;
+        // tile init: depth from the fused clear or the target; no fragment yet
+        for (uint32_t i = lane; i < npx; i += WAVE) {
+            const uint32_t r = i / ncols, c = i - r * ncols;
+            if (has_depth)
+                s_dep[c * LD + r] = F.clear_depth ? F.clear_depth_value
+                                                  : F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c];
+            s_tag[c * LD + r] = TAG_NONE;
+        }
         __syncthreads();
 
-        const int32_t row = (int32_t)(y0 + lane);
-        const bool row_in = lane < nrows;
         unsigned long long fi_acc = 0, fo_acc = 0;
         uint32_t errs = 0;
+        uint32_t fq_head = 0, fq_tail = 0;  // ring indices (wave-uniform)
 
-        for (uint32_t ri = 0; ri < nruns; ++ri) {
-            const Run run = B.runs[off + ri];
-            const uint64_t g = (uint64_t)run.group * WAVE + lane;
-            const bool has = (run.mask >> lane) & 1ull;
-            const uint32_t h = has ? B.hdr[g] : 0u;
-            const uint32_t nf = h & 7u;
-            uint32_t F_total;
-            const uint32_t pre = wave_excl_scan(nf, &F_total);
-            for (uint32_t j = 0; j < nf; ++j) s_fan[pre + j] = (uint16_t)((lane << 3) | j);
-            __syncthreads();
-            for (uint32_t sub = 0; sub < F_total; sub += WAVE) {
-                const uint32_t idx = sub + lane;
-                if (idx < F_total) {
-                    const uint32_t e = s_fan[idx];
-                    const uint32_t li = e >> 3, j = e & 7u;
-                    const uint64_t gg = (uint64_t)run.group * WAVE + li;
-                    const ScrTri* st = (j == 0) ? &B.scr[gg] : &B.extra[(B.hdr[gg] >> 3) + j - 1];
-                    float v[3][6];
-                    for (int k = 0; k < 3; ++k)
-                        for (int c = 0; c < 6; ++c) v[k][c] = st->v[k][c];
-                    const DrawDesc* D = &B.draws[st->draw];
-                    Slot& S = s_slot[lane];
-                    const int na = kind_attrs(D->kind);
-                    tri_setup(S, v[0], v[1], v[2], na);
-                    S.kind = D->kind;
-                    S.flat = D->flat_rgba;
-                    S.tex = D->tex;
-                    S.tw = D->tex_w;
-                    S.th = D->tex_h;
-                    S.tstride = D->tex_stride;
-                    const uint32_t rr0 = B.rect[2 * gg], rr1 = B.rect[2 * gg + 1];
-                    S.rx0 = (rr0 & 0xFFFF) * TW;
-                    S.rx1 = ((rr1 & 0xFFFF) + 1) * TW;
-                    // mark the tile rows this fan covers
-                    int32_t a = S.ylo - (int32_t)y0, b = S.yhi - (int32_t)y0;
-                    a = a < 0 ? 0 : a;
-                    b = b > (int32_t)nrows ? (int32_t)nrows : b;
-                    for (int32_t r = a; r < b; ++r) atomicOr(&s_rowmask[r], 1ull << lane);
-                }
+        for (uint32_t ri = 0; ri <= nruns; ++ri) {
+            // 1. append the next run's triangles (their fans) to the queue
+            if (ri < nruns) {
+                const Run run = B.runs[off + ri];
+                const uint64_t g = (uint64_t)run.group * WAVE + lane;
+                const bool has = (run.mask >> lane) & 1ull;
+                const uint32_t nf = has ? (B.hdr[g] & 7u) : 0u;
+                uint32_t tot;
+                const uint32_t pre = wave_excl_scan(nf, &tot);
+                for (uint32_t j = 0; j < nf; ++j) s_fq[(fq_tail + pre + j) % FQ_CAP] = ((uint32_t)g << 3) | j;
+                fq_tail += tot;
                 __syncthreads();
-                unsigned long long m = s_rowmask[lane];
-                s_rowmask[lane] = 0;
-                if (!row_in) m = 0;
-                while (m) {
-                    const uint32_t s = (uint32_t)(__ffsll((long long)m) - 1);
-                    m &= m - 1;
-                    const Slot& S = s_slot[s];
-                    const int na = kind_attrs(S.kind);
-                    for (int hh = 0; hh < 2; ++hh) {
-                        const Trap& T = S.t[hh];
-                        const int32_t k = row - (int32_t)T.y0i;
-                        if (k < 0 || (uint32_t)k >= T.n) continue;
-                        // ScanlineIter::next: left/right iterators stepped k times
-                        // (exactly; long runs through the binade fast-forward)
+            }
+            // 2. consume full batches, and at the end whatever is left
+            while (fq_tail - fq_head >= (ri < nruns ? (uint32_t)WAVE : 1u)) {
+                const uint32_t avail = fq_tail - fq_head;
+                const uint32_t nb = avail < (uint32_t)WAVE ? avail : (uint32_t)WAVE;
+                // ---- setup: lane = triangle
+                const bool act = lane < nb;
+                Slot S;
+                uint32_t rows_in = 0;
+                int32_t ra = 0, rb = 0;
+                uint32_t rx0b = 0, rx1b = 0;
+                int na = 0;
+                if (act) {
+                    const uint32_t e = s_fq[(fq_head + lane) % FQ_CAP];
+                    const uint64_t gg = e >> 3;
+                    const uint32_t j = e & 7u;
+                    const ScrTri* st = (j == 0) ? &B.scr[gg] : &B.extra[(B.hdr[gg] >> 3) + j - 1];
+                    float va[6], vb[6], vc[6];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        va[c] = st->v[0][c];
+                        vb[c] = st->v[1][c];
+                        vc[c] = st->v[2][c];
+                    }
+                    const uint32_t di = st->draw;
+                    const DrawDesc& D = B.draws[di];
+                    const uint32_t kind = D.kind;
+                    na = kind_attrs(kind);
+                    tri_setup(S, va, vb, vc, na);
+                    SlotP& P = s_slot[lane];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        P.dv[h][0] = S.t[h].dvz;
+                        for (int c = 0; c < 3; ++c) P.dv[h][1 + c] = S.t[h].dva[c];
+                    }
+                    P.kind = kind;
+                    P.draw = di;
+                    if (kind == KIND_FLAT) {
+                        const uint32_t fc = D.flat_rgba;
+                        P.p0 = pack_pixel(F.fmt, fc & 0xFF, (fc >> 8) & 0xFF, (fc >> 16) & 0xFF, fc >> 24);
+                    } else {
+                        P.p0 = D.tex_w;
+                    }
+                    P.p1 = D.tex_h;
+                    P.p2 = D.tex_stride;
+                    const uint32_t rr0 = B.rect[2 * gg], rr1 = B.rect[2 * gg + 1];
+                    rx0b = (rr0 & 0xFFFF) * TW;
+                    rx1b = ((rr1 & 0xFFFF) + 1) * TW;
+                    ra = S.ylo - (int32_t)y0;
+                    rb = S.yhi - (int32_t)y0;
+                    ra = ra < 0 ? 0 : ra;
+                    rb = rb > (int32_t)nrows ? (int32_t)nrows : rb;
+                    rows_in = rb > ra ? (uint32_t)(rb - ra) : 0u;
+                }
+                uint32_t rec_tot;
+                const uint32_t rec_base = wave_excl_scan(rows_in, &rec_tot);
+                // take the leading triangles whose records fit (lane 0 always
+                // fits: it has at most TILE_H rows)
+                const bool fits = rec_base + rows_in <= (uint32_t)REC_CAP;
+                const unsigned long long fit_mask = __ballot(act && fits);
+                const unsigned long long nf_mask = ~fit_mask;
+                const uint32_t nuse = nf_mask == 0ull ? (uint32_t)WAVE : (uint32_t)(__ffsll((long long)nf_mask) - 1);
+                const bool use = lane < nuse;
+                if (use) {
+                    SlotP& P = s_slot[lane];
+                    P.rec_base = rec_base;
+                    P.ra = ra;
+                    P.rb = rows_in ? rb : ra;
+                }
+                // ---- edges: lane = triangle, walk its rows in the tile
+                if (use && rows_in) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const Trap& T = S.t[h];
+                        if (T.n == 0) continue;
+                        const int32_t ty0 = (int32_t)T.y0i;  // absolute first row
+                        int32_t a = ty0 > (int32_t)y0 + ra ? ty0 : (int32_t)y0 + ra;
+                        int32_t b = ty0 + (int32_t)T.n;
+                        if (b > (int32_t)y0 + rb) b = (int32_t)y0 + rb;
+                        if (a >= b) continue;
+                        const uint32_t k0 = (uint32_t)(a - ty0);
                         float lx, lz, l0, l1, l2, rx;
-                        if (k <= FF_MIN) {
-                            lx = T.lx; lz = T.lz; l0 = T.la[0]; l1 = T.la[1]; l2 = T.la[2];
-                            rx = T.rx;
-                            for (int32_t q = 0; q < k; ++q) {
+                        if (k0 <= (uint32_t)KMAX) {
+                            lx = T.lx; lz = T.lz; l0 = T.la[0]; l1 = T.la[1]; l2 = T.la[2]; rx = T.rx;
+                            for (uint32_t q = 0; q < k0; ++q) {
                                 lx = lx + T.dlx;
                                 lz = lz + T.dlz;
-                                if (na > 0) {
-                                    l0 = l0 + T.dla[0];
-                                    l1 = l1 + T.dla[1];
-                                    if (na > 2) l2 = l2 + T.dla[2];
-                                }
+                                l0 = l0 + T.dla[0];
+                                l1 = l1 + T.dla[1];
+                                l2 = l2 + T.dla[2];
                                 rx = rx + T.drx;
                             }
                         } else {
-                            lx = ff_add(T.lx, T.dlx, (uint32_t)k);
-                            lz = ff_add(T.lz, T.dlz, (uint32_t)k);
-                            l0 = na > 0 ? ff_add(T.la[0], T.dla[0], (uint32_t)k) : T.la[0];
-                            l1 = na > 0 ? ff_add(T.la[1], T.dla[1], (uint32_t)k) : T.la[1];
-                            l2 = na > 2 ? ff_add(T.la[2], T.dla[2], (uint32_t)k) : T.la[2];
-                            rx = ff_add(T.rx, T.drx, (uint32_t)k);
+                            lx = ff_add(T.lx, T.dlx, k0);
+                            lz = ff_add(T.lz, T.dlz, k0);
+                            l0 = na > 0 ? ff_add(T.la[0], T.dla[0], k0) : T.la[0];
+                            l1 = na > 0 ? ff_add(T.la[1], T.dla[1], k0) : T.la[1];
+                            l2 = na > 2 ? ff_add(T.la[2], T.dla[2], k0) : T.la[2];
+                            rx = ff_add(T.rx, T.drx, k0);
                         }
-                        const float rx0 = round_up_to_half(lx), rx1 = round_up_to_half(rx);
-                        // v0.lerp(&v0.step(&dv_dx), x0 - v0.x)
-                        const float tw_ = rx0 - lx;
-                        float z = lz + ((lz + T.dvz) - lz) * tw_;
-                        float c0 = l0 + ((l0 + T.dva[0]) - l0) * tw_;
-                        float c1 = l1 + ((l1 + T.dva[1]) - l1) * tw_;
-                        float c2 = l2 + ((l2 + T.dva[2]) - l2) * tw_;
-                        const uint32_t cnt = as_u32(rx1 - rx0);
-                        const uint64_t xs = as_usize(rx0);
-                        uint64_t xe = as_usize(rx1);
-                        if (xe < xs) xe = xs;
-                        if (xe > F.width || (uint32_t)row >= F.height) {
-                            errs |= ERRB_BOUNDS;
-                            continue;
-                        }
-                        if (xe > xs && (xs < S.rx0 || xe > S.rx1)) errs |= ERRB_MARGIN;
-                        // frags.i: the part of [xs, xe) inside this tile
-                        {
-                            const uint64_t a = xs > x0 ? xs : x0;
-                            const uint64_t b = xe < (uint64_t)(x0 + TW) ? xe : (uint64_t)(x0 + TW);
-                            if (b > a) fi_acc += b - a;
-                        }
-                        const uint64_t nfrag = (uint64_t)cnt < xe - xs ? (uint64_t)cnt : xe - xs;
-                        const int64_t j0 = (int64_t)x0 - (int64_t)xs > 0 ? (int64_t)x0 - (int64_t)xs : 0;
-                        int64_t j1 = (int64_t)x0 + TW - (int64_t)xs;
-                        if (j1 > (int64_t)nfrag) j1 = (int64_t)nfrag;
-                        if (j0 >= j1) continue;
-                        // the fragment iterator stepped j0 times (Scanline::fragments)
-                        if (j0 <= FF_MIN) {
-                            for (int64_t q = 0; q < j0; ++q) {
-                                z = z + T.dvz;
-                                if (na > 0) {
-                                    c0 = c0 + T.dva[0];
-                                    c1 = c1 + T.dva[1];
-                                    if (na > 2) c2 = c2 + T.dva[2];
-                                }
+                        for (int32_t y = a; y < b; ++y) {
+                            const float rx0 = round_up_to_half(lx), rx1 = round_up_to_half(rx);
+                            // v0.lerp(&v0.step(&dv_dx), x0 - v0.x)
+                            const float tw_ = rx0 - lx;
+                            SpanRec R;
+                            R.z = lz + ((lz + T.dvz) - lz) * tw_;
+                            R.a0 = l0 + ((l0 + T.dva[0]) - l0) * tw_;
+                            R.a1 = l1 + ((l1 + T.dva[1]) - l1) * tw_;
+                            R.a2 = l2 + ((l2 + T.dva[2]) - l2) * tw_;
+                            const uint32_t cnt = as_u32(rx1 - rx0);
+                            const uint64_t xs = as_usize(rx0);
+                            uint64_t xe = as_usize(rx1);
+                            if (xe < xs) xe = xs;
+                            uint32_t nfrag = 0, xsr = 0;
+                            if (xe > F.width || (uint32_t)y >= F.height) {
+                                errs |= ERRB_BOUNDS;
+                            } else {
+                                if (xe > xs && (xs < rx0b || xe > rx1b)) errs |= ERRB_MARGIN;
+                                // frags.i: the part of [xs, xe) inside this tile
+                                const uint64_t pa = xs > x0 ? xs : x0;
+                                const uint64_t pb = xe < (uint64_t)(x0 + TW) ? xe : (uint64_t)(x0 + TW);
+                                if (pb > pa) fi_acc += pb - pa;
+                                nfrag = (uint64_t)cnt < xe - xs ? cnt : (uint32_t)(xe - xs);
+                                xsr = (uint32_t)xs;
                             }
-                        } else {
-                            z = ff_add(z, T.dvz, (uint32_t)j0);
-                            if (na > 0) {
-                                c0 = ff_add(c0, T.dva[0], (uint32_t)j0);
-                                c1 = ff_add(c1, T.dva[1], (uint32_t)j0);
-                                if (na > 2) c2 = ff_add(c2, T.dva[2], (uint32_t)j0);
-                            }
+                            R.xs_nfrag_h = xsr | (nfrag << 15) | ((uint32_t)h << 31);
+                            s_rec[rec_base + (uint32_t)(y - (int32_t)y0 - ra)] = R;
+                            lx = lx + T.dlx;
+                            lz = lz + T.dlz;
+                            l0 = l0 + T.dla[0];
+                            l1 = l1 + T.dla[1];
+                            l2 = l2 + T.dla[2];
+                            rx = rx + T.drx;
                         }
-                        const uint32_t lrow = lane;
-                        const uint32_t col0 = (uint32_t)((int64_t)xs + j0 - (int64_t)x0);
-                        const uint32_t npx = (uint32_t)(j1 - j0);
-                        // pixels in chunks of CHUNK: step the varyings, test depth,
-                        // then fetch all texels of the chunk before writing any
-                        for (uint32_t q0 = 0; q0 < npx; q0 += CHUNK) {
-                            float zz[CHUNK], v0[CHUNK], v1[CHUNK], v2[CHUNK];
-                            bool pass[CHUNK];
+                    }
+                }
+                // row masks: lane r gets the triangles covering tile row r
+                unsigned long long mymask = 0;
+                for (int r = 0; r < TILE_H; ++r) {
+                    const unsigned long long m = __ballot(use && rows_in && ra <= r && r < rb);
+                    if ((int)lane == r) mymask = m;
+                }
+                __syncthreads();
+                // ---- spans: lane = row
+                const uint32_t lrow = lane;
+                while (mymask) {
+                    const uint32_t f = (uint32_t)(__ffsll((long long)mymask) - 1);
+                    mymask &= mymask - 1;
+                    const SlotP& P = s_slot[f];
+                    const SpanRec R = s_rec[P.rec_base + (uint32_t)((int32_t)lrow - P.ra)];
+                    const uint32_t xs = R.xs_nfrag_h & 0x7FFFu;
+                    const uint32_t nfrag = (R.xs_nfrag_h >> 15) & 0xFFFFu;
+                    const uint32_t h = R.xs_nfrag_h >> 31;
+                    const int32_t j0 = (int32_t)x0 - (int32_t)xs > 0 ? (int32_t)x0 - (int32_t)xs : 0;
+                    int32_t j1 = (int32_t)x0 + TW - (int32_t)xs;
+                    if (j1 > (int32_t)nfrag) j1 = (int32_t)nfrag;
+                    if (j0 >= j1) continue;
+                    const float dz = P.dv[h][0], d0 = P.dv[h][1], d1 = P.dv[h][2], d2 = P.dv[h][3];
+                    const int pna = kind_attrs(P.kind);
+                    float z = R.z, c0 = R.a0, c1 = R.a1, c2 = R.a2;
+                    // the fragment iterator stepped j0 times (Scanline::fragments)
+                    if (j0 <= JMAX) {
+                        for (int32_t q = 0; q < j0; ++q) {
+                            z = z + dz;
+                            c0 = c0 + d0;
+                            c1 = c1 + d1;
+                            c2 = c2 + d2;
+                        }
+                    } else {
+                        z = ff_add(z, dz, (uint32_t)j0);
+                        if (pna > 0) {
+                            c0 = ff_add(c0, d0, (uint32_t)j0);
+                            c1 = ff_add(c1, d1, (uint32_t)j0);
+                            if (pna > 2) c2 = ff_add(c2, d2, (uint32_t)j0);
+                        }
+                    }
+                    uint32_t li = ((uint32_t)((int32_t)xs + j0) - x0) * LD + lrow;
+                    for (int32_t q0 = j0; q0 < j1; q0 += CHUNK, li += CHUNK * LD) {
+                        const int32_t rem = j1 - q0;
+                        float zz[CHUNK], v0[CHUNK], v1[CHUNK], v2[CHUNK], cur[CHUNK];
 #pragma unroll
-                            for (int p = 0; p < CHUNK; ++p) {
-                                zz[p] = z;
-                                v0[p] = c0;
-                                v1[p] = c1;
-                                v2[p] = c2;
-                                z = z + T.dvz;
-                                if (na > 0) {
-                                    c0 = c0 + T.dva[0];
-                                    c1 = c1 + T.dva[1];
-                                    if (na > 2) c2 = c2 + T.dva[2];
-                                }
-                            }
-                            uint32_t px[CHUNK];
+                        for (int p = 0; p < CHUNK; ++p) {
+                            zz[p] = z;
+                            v0[p] = c0;
+                            v1[p] = c1;
+                            v2[p] = c2;
+                            z = z + dz;
+                            c0 = c0 + d0;
+                            c1 = c1 + d1;
+                            c2 = c2 + d2;
+                        }
+                        // the pixels of one chunk are distinct: read all depths first
+                        if (has_depth) {
 #pragma unroll
-                            for (int p = 0; p < CHUNK; ++p) {
-                                const bool in = q0 + p < npx;
-                                const uint32_t li = (col0 + q0 + p) * LD + lrow;
-                                pass[p] = in && (!has_depth || depth_pass(F.depth_test, zz[p], s_dep[in ? li : lrow]));
-                                px[p] = pass[p] ? shade(S, F.fmt, zz[p], v0[p], v1[p], v2[p]) : 0u;
-                            }
+                            for (int p = 0; p < CHUNK; ++p) cur[p] = s_dep[p < rem ? li + p * LD : li];
+                        }
 #pragma unroll
-                            for (int p = 0; p < CHUNK; ++p) {
-                                if (!pass[p]) continue;
-                                const uint32_t li = (col0 + q0 + p) * LD + lrow;
+                        for (int p = 0; p < CHUNK; ++p) {
+                            if (p < rem && (!has_depth || depth_pass(F.depth_test, zz[p], cur[p]))) {
+                                const uint32_t a = li + p * LD;
                                 if (F.color_write) {
                                     ++fo_acc;
-                                    s_col[li] = px[p];
+                                    uint32_t tg, py;
+                                    frag_payload(P, F.fmt, zz[p], v0[p], v1[p], v2[p], tg, py);
+                                    s_tag[a] = tg;
+                                    s_pay[a] = py;
                                 }
-                                if (has_depth && F.depth_write) s_dep[li] = zz[p];
+                                if (has_depth && F.depth_write) s_dep[a] = zz[p];
                             }
                         }
                     }
                 }
                 __syncthreads();
+                fq_head += nuse;
             }
         }
 
-        // write the tile back once
-        for (uint32_t r = 0; r < nrows; ++r)
-            for (uint32_t c = lane; c < ncols; c += WAVE) {
-                store_pixel(F, x0 + c, y0 + r, s_col[c * LD + r]);
-                if (has_depth) F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = s_dep[c * LD + r];
+        // write-back: fetch the texels, colour and depth to the target
+        for (uint32_t i = lane; i < npx; i += WAVE) {
+            const uint32_t r = i / ncols, c = i - r * ncols;
+            const uint32_t li = c * LD + r;
+            const uint32_t tg = s_tag[li];
+            if (tg == TAG_COLOR) {
+                store_pixel(F, x0 + c, y0 + r, s_pay[li]);
+            } else if (tg != TAG_NONE) {
+                const uint32_t t = ((const uint32_t*)B.draws[tg].tex)[s_pay[li]];
+                store_pixel(F, x0 + c, y0 + r, pack_pixel(F.fmt, t & 0xFF, (t >> 8) & 0xFF, (t >> 16) & 0xFF, t >> 24));
+            } else if (F.clear_color) {
+                store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
             }
+            if (has_depth) F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = s_dep[li];
+        }
         const unsigned long long fi_s = wave_sum_u64(fi_acc), fo_s = wave_sum_u64(fo_acc);
         if (lane == 0) {
             if (fi_s) atomicAdd(&B.stats[fi].frags_i, fi_s);
