@@ -40,7 +40,42 @@ __host__ __device__ inline int ff_binade(double a) {
     return e;
 }
 
+// Float-only fast path of ff_add: after one serial step s0 -> s1, if s0, s1
+// and s2 = fl(s1 + d) share sign and binade B = [lo, hi) and every exact sum
+// s1 + j*inc + d (j = 0 .. n-2) stays in B, the result is s1 + (n-1)*inc,
+// which fmaf computes exactly (it is a float of B or hi). The two end checks
+// are exact: y - lo and hi - y are floats for y in B (Sterbenz), and
+// fmaf(j, inc, s1) is exact whenever its value lies in B. Returns false
+// (leaving `out` unset) when the conditions do not hold.
+__host__ __device__ inline bool ff_add_fast(float s0, float d, uint32_t n, float& out) {
+    const float s1 = s0 + d, s2 = s1 + d;
+    uint32_t b0, b1, b2;
+    __builtin_memcpy(&b0, &s0, 4);
+    __builtin_memcpy(&b1, &s1, 4);
+    __builtin_memcpy(&b2, &s2, 4);
+    const uint32_t ex = (b1 >> 23) & 0xFFu;
+    // same sign and exponent field; magnitudes in [2^-90, 2^100); d finite; a real step
+    if (((b0 ^ b1) | (b1 ^ b2)) >= (1u << 23) || ex < 37u || ex > 226u || !(d - d == 0.0f) || s1 == s2)
+        return false;
+    const bool neg = (b1 >> 31) != 0;
+    const float a1 = neg ? -s1 : s1, da = neg ? -d : d;
+    const float inc = neg ? s1 - s2 : s2 - s1;  // exact: same binade
+    const uint32_t blo = ex << 23;
+    float lo;
+    __builtin_memcpy(&lo, &blo, 4);
+    const float hi = 2.0f * lo;
+    const float yl = fmaf((float)(n - 2), inc, a1);  // s_(n-1), exact when in B
+    if (!(yl >= lo && yl < hi && yl - lo >= -da && hi - yl > da)) return false;
+    const float r = fmaf((float)(n - 1), inc, a1);
+    out = neg ? -r : r;
+    return true;
+}
+
 __host__ __device__ inline float ff_add(float s, float d, uint32_t n) {
+    if (n > 3) {
+        float r;
+        if (ff_add_fast(s, d, n, r)) return r;
+    }
     while (n > 0) {
         if (n <= 3) {
             for (; n > 0; --n) s = s + d;
@@ -72,6 +107,12 @@ __host__ __device__ inline float ff_add(float s, float d, uint32_t n) {
             const double x = ss + (double)j * inc + dd;
             return x >= lo && x < hi;
         };
+        // common case: every remaining step stays in the binade (the exact
+        // sums are linear in j, so the first and the last decide)
+        if (ok(0) && ok(n - 1)) {
+            const double r = ss + (double)n * inc;
+            return neg ? -(float)r : (float)r;
+        }
         double jm;
         if (inc > 0.0)
             jm = ceil((hi - ss - dd) / inc);

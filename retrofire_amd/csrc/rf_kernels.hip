@@ -759,18 +759,18 @@ __device__ __forceinline__ uint32_t find_frame_tile(const FrameDesc* f, uint32_t
     return lo;
 }
 
-// Per-triangle data the row lanes read during the pixel phase.
+// Per-triangle data of a batch slot.
 struct SlotP {
     float dv[2][4];        // d(z, a0, a1, a2)/dx of the upper and lower trapezoid
     uint32_t kind;
     uint32_t p0, p1, p2;   // flat: packed colour; textured: width, height, row stride (texels)
-    uint32_t draw;
-    uint32_t rec_base;     // first span record
-    int32_t ra, rb;        // tile rows [ra, rb) with a record
+    uint32_t tag;          // draw - the tile's draw base
+    int32_t ra;            // first tile row with a span record
+    uint32_t rec_base;     // its record
+    uint32_t pad;
 };
-
-// Span record of one (triangle, row): the varyings at the span's first pixel
-// (after the x0 adjustment of ScanlineIter::next), its pixel range and
+// Span record of one (triangle, tile row): the varyings at the span's first
+// pixel (after the x0 adjustment of ScanlineIter::next), its start and
 // fragment count.
 struct SpanRec {
     float z, a0, a1, a2;
@@ -781,81 +781,106 @@ struct SpanRec {
 #define RF_REC_CAP 384
 #endif
 constexpr int REC_CAP = RF_REC_CAP;  // span records per batch
-constexpr int FQ_CAP = 512;     // fan queue (ring); one run adds at most 64 * 7 fans
-constexpr int KMAX = 160;       // vertical steps above which the fast-forward is used
-constexpr int JMAX = 200;       // horizontal steps above which the fast-forward is used
+constexpr int FQ_CAP = 512;          // fan queue (ring); one run adds at most 64 * 7 fans
+constexpr int CELL_W = 8;            // pixel columns per ordering cell
+#ifndef RF_KMAX
+#define RF_KMAX 160
+#endif
+#ifndef RF_JSER
+#define RF_JSER 64
+#endif
+constexpr int KMAX = RF_KMAX;  // row steps taken serially; more use the exact fast-forward
+constexpr int JSER = RF_JSER;  // pixel steps taken serially; more use the exact fast-forward
+#ifndef RF_REFILL_MIN
+#define RF_REFILL_MIN 1
+#endif
+constexpr int REFILL_MIN = RF_REFILL_MIN;  // free lanes that trigger a refill while others still work
 
 // Tile rasterizer. One wave owns one tile (TILE_H rows x TW columns) at a
-// time; the tile's depth and a per-pixel record of its last passing fragment
-// live in LDS. Triangles are consumed in submission order in batches of up to
+// time; the tile's depth and each pixel's colour source live in LDS.
+// Triangles (clip fans) are consumed in submission order in batches of up to
 // 64:
 //   setup  lane = triangle: tri_fill/scan setup (raster.rs:185-302);
 //   edges  lane = triangle: walks its rows inside the tile with the serial
 //          edge recurrence (ScanlineIter::next, raster.rs:84-113), entering
 //          the tile through the exact fast-forward when it starts far above,
-//          and writes one span record per row;
-//   spans  lane = row: walks the span records of its row in triangle order,
-//          stepping the varyings pixel by pixel (Scanline::fragments,
-//          raster.rs:61-68) and applying the depth test and depth write of
-//          rasterize_fb (target.rs:163-198). Each pixel row belongs to one
-//          lane, so every pixel sees its fragments in submission order.
+//          and writes one span record per row, in triangle-major order; a
+//          record's index is its order key;
+//   spans  lanes take records in key order and keep one until it is done:
+//          the varyings are stepped serially pixel by pixel from the span's
+//          start (Scanline::fragments, raster.rs:61-68; the exact
+//          fast-forward enters a tile far from the start). In rounds, every
+//          record claims the (row, 8-column) cells its remaining pixels touch
+//          with an LDS atomic minimum of its key, and a record whose next 8
+//          pixels lie in cells it holds applies them: depth test, depth write
+//          and colour source (rasterize_fb, target.rs:163-198). A cell held by
+//          a record blocks every later one, so each pixel sees its fragments
+//          in submission order; the record with the smallest key always
+//          proceeds.
 // A passing fragment that writes colour leaves in the pixel's record what its
 // colour is made of: the colour itself for the flat and Gouraud shaders, the
-// draw and the sampled texel's index for the textured ones (the samplers'
-// coordinate arithmetic runs in the span walk; only the texel fetch waits).
-// The colour a pixel ends with is that of its last passing fragment, so the
-// fetches run once per written pixel at the end of the tile, all pixels in
-// parallel, and colour and depth go straight to the target.
-constexpr uint32_t TAG_NONE = 0xFFFFFFFFu;   // no fragment wrote colour
-constexpr uint32_t TAG_COLOR = 0xFFFFFFFEu;  // payload is the colour; else tag = draw, payload = texel
+// draw (relative to the tile's first draw) and the sampled texel's index for
+// the textured ones (the samplers' coordinate arithmetic runs here; only the
+// texel fetch waits). The colour a pixel ends with is that of its last
+// passing fragment, so the fetches run once per written pixel at the end of
+// the tile, all pixels in parallel, and colour and depth go straight to the
+// target.
+constexpr uint16_t TAG_NONE = 0xFFFFu;   // no fragment wrote colour
+constexpr uint16_t TAG_COLOR = 0xFFFEu;  // payload is the colour; else tag = draw - base, payload = texel
 
-__device__ __forceinline__ void frag_payload(const SlotP& P, uint32_t fmt, float z, float a0, float a1,
-                                             float a2, uint32_t& tag, uint32_t& pay) {
-    switch (P.kind) {
-        case KIND_FLAT:
-            tag = TAG_COLOR;
-            pay = P.p0;
-            return;
-        case KIND_GOURAUD:  // Color3f::to_color4 (color.rs:253-263)
-            tag = TAG_COLOR;
-            pay = pack_pixel(fmt, as_u8(256.0f * a0), as_u8(256.0f * a1), as_u8(256.0f * a2), 0xFF);
-            return;
-        default: {
-            // Scanline::fragments z_div (raster.rs:61-68, vec.rs:666-676)
-            const float u = a0 / z, v = a1 / z;
-            const float tw = (float)P.p0, th = (float)P.p1;
-            uint32_t ui, vi;
-            if (P.kind == KIND_TEX_CLAMP) {  // SamplerClamp (tex.rs:280-303)
-                float su = u * tw, sv = v * th;
-                const float mu = tw - 1.0f, mv = th - 1.0f;
-                if (su < 0.0f) su = 0.0f;
-                if (su > mu) su = mu;
-                if (sv < 0.0f) sv = 0.0f;
-                if (sv > mv) sv = mv;
-                ui = as_u32(floorf(su));
-                vi = as_u32(floorf(sv));
-            } else {  // SamplerRepeatPot (tex.rs:240-266)
-                ui = (uint32_t)as_i32(floorf(tw * u)) & (P.p0 - 1);
-                vi = (uint32_t)as_i32(floorf(th * v)) & (P.p1 - 1);
-            }
-            tag = P.draw;
-            pay = vi * P.p2 + ui;
+__device__ __forceinline__ uint32_t texel_pixel(const BatchDesc& B, uint32_t fmt, uint32_t draw, uint32_t texel) {
+    const uint32_t t = ((const uint32_t*)B.draws[draw].tex)[texel];
+    return pack_pixel(fmt, t & 0xFF, (t >> 8) & 0xFF, (t >> 16) & 0xFF, t >> 24);
+}
+
+// n serial additions of d to each of k values, serially for small n and
+// through the exact fast-forward otherwise (only the first nlive values).
+template <int K>
+__device__ __forceinline__ void advance(float (&v)[K], const float (&d)[K], uint32_t n, uint32_t nser, int nlive) {
+    if (n <= nser) {
+        for (uint32_t q = 0; q < n; ++q) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) v[i] = v[i] + d[i];
         }
+    } else {
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if (i < nlive) v[i] = ff_add(v[i], d[i], n);
     }
+}
+
+__device__ __forceinline__ int32_t wave_incl_max_i32(int32_t v) {
+    const uint32_t lane = lane_id();
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int32_t y = __shfl_up(v, o, WAVE);
+        if (lane >= (uint32_t)o) v = y > v ? y : v;
+    }
+    return v;
+}
+
+// The depth test of Context::depth_test (ctx.rs:86-89) without branches.
+__device__ __forceinline__ bool depth_pass_u(int32_t dt, float nz, float cur) {
+    return (dt == DT_NONE) | ((dt == DT_LESS) & (cur < nz)) | ((dt == DT_EQUAL) & (cur == nz)) |
+           ((dt == DT_GREATER) & (cur > nz));
 }
 
 template <int TW>
 __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
     constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
     constexpr int NPX = TW * LD;
+    constexpr int NCG = TW / CELL_W;
     __shared__ float s_dep[NPX];
-    __shared__ uint32_t s_tag[NPX], s_pay[NPX];  // the pixel's colour source
+    __shared__ uint32_t s_pay[NPX];  // the pixel's colour source: payload
+    __shared__ uint16_t s_tag[NPX];  //   and tag
     __shared__ SlotP s_slot[WAVE];
     __shared__ SpanRec s_rec[REC_CAP];
-    __shared__ uint32_t s_fq[FQ_CAP];   // g << 3 | fan
+    __shared__ uint32_t s_fq[FQ_CAP];  // g << 3 | fan
+    __shared__ int8_t s_own[REC_CAP];  // slot whose records start here, or -1
+    __shared__ uint32_t s_cell[TILE_H * NCG];
     __shared__ uint32_t s_tile;
 
     const uint32_t lane = lane_id();
+    for (uint32_t i = lane; i < (uint32_t)(TILE_H * NCG); i += WAVE) s_cell[i] = 0xFFFFFFFFu;
     for (;;) {
         if (lane == 0) s_tile = atomicAdd(B.tile_counter, 1u);
         __syncthreads();
@@ -872,6 +897,8 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
         if (off >= B.runs_cap) nruns = 0;
         else if (off + nruns > B.runs_cap) nruns = B.runs_cap - off;
         const bool has_depth = F.depth != nullptr;
+        const bool dwrite = has_depth && F.depth_write;
+        const int32_t dtest = has_depth ? F.depth_test : (int32_t)DT_NONE;
         const uint32_t ncols = min((uint32_t)TW, F.width - x0);
         const uint32_t nrows = min((uint32_t)TILE_H, F.height - y0);
         const uint32_t npx = ncols * nrows;
@@ -900,6 +927,8 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
         unsigned long long fi_acc = 0, fo_acc = 0;
         uint32_t errs = 0;
         uint32_t fq_head = 0, fq_tail = 0;  // ring indices (wave-uniform)
+        uint32_t dbase = 0;                 // draw of the tile's first triangle
+        bool dbase_set = false;
 
         for (uint32_t ri = 0; ri <= nruns; ++ri) {
             // 1. append the next run's triangles (their fans) to the queue
@@ -921,7 +950,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                 // ---- setup: lane = triangle
                 const bool act = lane < nb;
                 Slot S;
-                uint32_t rows_in = 0;
+                uint32_t rows_in = 0, di = 0, kind = 0;
                 int32_t ra = 0, rb = 0;
                 uint32_t rx0b = 0, rx1b = 0;
                 int na = 0;
@@ -937,27 +966,10 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         vb[c] = st->v[1][c];
                         vc[c] = st->v[2][c];
                     }
-                    const uint32_t di = st->draw;
-                    const DrawDesc& D = B.draws[di];
-                    const uint32_t kind = D.kind;
+                    di = st->draw;
+                    kind = B.draws[di].kind;
                     na = kind_attrs(kind);
                     tri_setup(S, va, vb, vc, na);
-                    SlotP& P = s_slot[lane];
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        P.dv[h][0] = S.t[h].dvz;
-                        for (int c = 0; c < 3; ++c) P.dv[h][1 + c] = S.t[h].dva[c];
-                    }
-                    P.kind = kind;
-                    P.draw = di;
-                    if (kind == KIND_FLAT) {
-                        const uint32_t fc = D.flat_rgba;
-                        P.p0 = pack_pixel(F.fmt, fc & 0xFF, (fc >> 8) & 0xFF, (fc >> 16) & 0xFF, fc >> 24);
-                    } else {
-                        P.p0 = D.tex_w;
-                    }
-                    P.p1 = D.tex_h;
-                    P.p2 = D.tex_stride;
                     const uint32_t rr0 = B.rect[2 * gg], rr1 = B.rect[2 * gg + 1];
                     rx0b = (rr0 & 0xFFFF) * TW;
                     rx1b = ((rr1 & 0xFFFF) + 1) * TW;
@@ -972,15 +984,52 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                 // take the leading triangles whose records fit (lane 0 always
                 // fits: it has at most TILE_H rows)
                 const bool fits = rec_base + rows_in <= (uint32_t)REC_CAP;
-                const unsigned long long fit_mask = __ballot(act && fits);
-                const unsigned long long nf_mask = ~fit_mask;
+                const unsigned long long nf_mask = ~__ballot(act && fits);
                 const uint32_t nuse = nf_mask == 0ull ? (uint32_t)WAVE : (uint32_t)(__ffsll((long long)nf_mask) - 1);
                 const bool use = lane < nuse;
+                const uint32_t nrec = __shfl(rec_base + rows_in, nuse - 1, WAVE);
+                // draw tags are relative to the tile's first draw (draws only grow
+                // along the triangle order); resolve pending texels before the
+                // range would overflow
+                const uint32_t dfirst = __shfl(di, 0, WAVE), dlast = __shfl(di, nuse - 1, WAVE);
+                if (!dbase_set) {
+                    dbase = dfirst;
+                    dbase_set = true;
+                } else if (dlast - dbase >= (uint32_t)TAG_COLOR) {
+                    for (uint32_t i = lane; i < npx; i += WAVE) {
+                        const uint32_t r = i / ncols, c = i - r * ncols;
+                        const uint32_t li = c * LD + r;
+                        const uint32_t tg = s_tag[li];
+                        if (tg < TAG_COLOR) {
+                            s_pay[li] = texel_pixel(B, F.fmt, dbase + tg, s_pay[li]);
+                            s_tag[li] = TAG_COLOR;
+                        }
+                    }
+                    dbase = dfirst;
+                }
+                for (uint32_t i = lane; i < nrec; i += WAVE) s_own[i] = -1;
+                __syncthreads();
                 if (use) {
                     SlotP& P = s_slot[lane];
-                    P.rec_base = rec_base;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        P.dv[h][0] = S.t[h].dvz;
+                        for (int c = 0; c < 3; ++c) P.dv[h][1 + c] = S.t[h].dva[c];
+                    }
+                    P.kind = kind;
+                    const DrawDesc& D = B.draws[di];
+                    if (kind == KIND_FLAT) {
+                        const uint32_t fc = D.flat_rgba;
+                        P.p0 = pack_pixel(F.fmt, fc & 0xFF, (fc >> 8) & 0xFF, (fc >> 16) & 0xFF, fc >> 24);
+                    } else {
+                        P.p0 = D.tex_w;
+                    }
+                    P.p1 = D.tex_h;
+                    P.p2 = D.tex_stride;
+                    P.tag = di - dbase;
                     P.ra = ra;
-                    P.rb = rows_in ? rb : ra;
+                    P.rec_base = rec_base;
+                    if (rows_in) s_own[rec_base] = (int8_t)lane;
                 }
                 // ---- edges: lane = triangle, walk its rows in the tile
                 if (use && rows_in) {
@@ -993,26 +1042,10 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         int32_t b = ty0 + (int32_t)T.n;
                         if (b > (int32_t)y0 + rb) b = (int32_t)y0 + rb;
                         if (a >= b) continue;
-                        const uint32_t k0 = (uint32_t)(a - ty0);
-                        float lx, lz, l0, l1, l2, rx;
-                        if (k0 <= (uint32_t)KMAX) {
-                            lx = T.lx; lz = T.lz; l0 = T.la[0]; l1 = T.la[1]; l2 = T.la[2]; rx = T.rx;
-                            for (uint32_t q = 0; q < k0; ++q) {
-                                lx = lx + T.dlx;
-                                lz = lz + T.dlz;
-                                l0 = l0 + T.dla[0];
-                                l1 = l1 + T.dla[1];
-                                l2 = l2 + T.dla[2];
-                                rx = rx + T.drx;
-                            }
-                        } else {
-                            lx = ff_add(T.lx, T.dlx, k0);
-                            lz = ff_add(T.lz, T.dlz, k0);
-                            l0 = na > 0 ? ff_add(T.la[0], T.dla[0], k0) : T.la[0];
-                            l1 = na > 0 ? ff_add(T.la[1], T.dla[1], k0) : T.la[1];
-                            l2 = na > 2 ? ff_add(T.la[2], T.dla[2], k0) : T.la[2];
-                            rx = ff_add(T.rx, T.drx, k0);
-                        }
+                        float ev[6] = {T.lx, T.rx, T.lz, T.la[0], T.la[1], T.la[2]};
+                        const float ed[6] = {T.dlx, T.drx, T.dlz, T.dla[0], T.dla[1], T.dla[2]};
+                        advance<6>(ev, ed, (uint32_t)(a - ty0), KMAX, 3 + na);
+                        float lx = ev[0], rx = ev[1], lz = ev[2], l0 = ev[3], l1 = ev[4], l2 = ev[5];
                         for (int32_t y = a; y < b; ++y) {
                             const float rx0 = round_up_to_half(lx), rx1 = round_up_to_half(rx);
                             // v0.lerp(&v0.step(&dv_dx), x0 - v0.x)
@@ -1049,50 +1082,82 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         }
                     }
                 }
-                // row masks: lane r gets the triangles covering tile row r
-                unsigned long long mymask = 0;
-                for (int r = 0; r < TILE_H; ++r) {
-                    const unsigned long long m = __ballot(use && rows_in && ra <= r && r < rb);
-                    if ((int)lane == r) mymask = m;
-                }
                 __syncthreads();
-                // ---- spans: lane = row
-                const uint32_t lrow = lane;
-                while (mymask) {
-                    const uint32_t f = (uint32_t)(__ffsll((long long)mymask) - 1);
-                    mymask &= mymask - 1;
-                    const SlotP& P = s_slot[f];
-                    const SpanRec R = s_rec[P.rec_base + (uint32_t)((int32_t)lrow - P.ra)];
-                    const uint32_t xs = R.xs_nfrag_h & 0x7FFFu;
-                    const uint32_t nfrag = (R.xs_nfrag_h >> 15) & 0xFFFFu;
-                    const uint32_t h = R.xs_nfrag_h >> 31;
-                    const int32_t j0 = (int32_t)x0 - (int32_t)xs > 0 ? (int32_t)x0 - (int32_t)xs : 0;
-                    int32_t j1 = (int32_t)x0 + TW - (int32_t)xs;
-                    if (j1 > (int32_t)nfrag) j1 = (int32_t)nfrag;
-                    if (j0 >= j1) continue;
-                    const float dz = P.dv[h][0], d0 = P.dv[h][1], d1 = P.dv[h][2], d2 = P.dv[h][3];
-                    const int pna = kind_attrs(P.kind);
-                    float z = R.z, c0 = R.a0, c1 = R.a1, c2 = R.a2;
-                    // the fragment iterator stepped j0 times (Scanline::fragments)
-                    if (j0 <= JMAX) {
-                        for (int32_t q = 0; q < j0; ++q) {
-                            z = z + dz;
-                            c0 = c0 + d0;
-                            c1 = c1 + d1;
-                            c2 = c2 + d2;
-                        }
-                    } else {
-                        z = ff_add(z, dz, (uint32_t)j0);
-                        if (pna > 0) {
-                            c0 = ff_add(c0, d0, (uint32_t)j0);
-                            c1 = ff_add(c1, d1, (uint32_t)j0);
-                            if (pna > 2) c2 = ff_add(c2, d2, (uint32_t)j0);
+
+                // ---- spans
+                uint32_t next = 0;   // next record to hand out (wave-uniform)
+                int32_t carry = -1;  // slot of the last record handed out
+                bool have = false;
+                uint32_t key = 0, row = 0, ikind = 0, ip0 = 0, ip1 = 0, ip2 = 0, itag = 0;
+                float dz = 0, d0 = 0, d1 = 0, d2 = 0, z = 0, c0 = 0, c1 = 0, c2 = 0;
+                int32_t xs = 0, jc = 0, j1 = 0;
+                for (;;) {
+                    const unsigned long long need = __ballot(!have);
+                    if (next < nrec && (uint32_t)__popcll(need) >= (uint32_t)REFILL_MIN) {
+                        uint32_t cnt;
+                        const uint32_t rank = wave_excl_scan(have ? 0u : 1u, &cnt);
+                        const uint32_t k = next + rank;
+                        const bool valid = !have && k < nrec;
+                        int32_t f = wave_incl_max_i32(valid ? (int32_t)s_own[k] : -1);
+                        f = f > carry ? f : carry;
+                        carry = __shfl(f, WAVE - 1, WAVE);  // the scan is non-decreasing
+                        next = next + cnt < nrec ? next + cnt : nrec;
+                        if (valid) {
+                            const SpanRec R = s_rec[k];
+                            const SlotP& P = s_slot[f];
+                            xs = (int32_t)(R.xs_nfrag_h & 0x7FFFu);
+                            const int32_t nfrag = (int32_t)((R.xs_nfrag_h >> 15) & 0xFFFFu);
+                            const uint32_t h = R.xs_nfrag_h >> 31;
+                            const int32_t jj0 = (int32_t)x0 - xs > 0 ? (int32_t)x0 - xs : 0;
+                            int32_t jj1 = (int32_t)x0 + TW - xs;
+                            if (jj1 > nfrag) jj1 = nfrag;
+                            if (jj0 < jj1) {
+                                ikind = P.kind;
+                                dz = P.dv[h][0];
+                                d0 = P.dv[h][1];
+                                d1 = P.dv[h][2];
+                                d2 = P.dv[h][3];
+                                float vv[4] = {R.z, R.a0, R.a1, R.a2};
+                                const float dd[4] = {dz, d0, d1, d2};
+                                // the fragment iterator stepped jj0 times (Scanline::fragments)
+                                advance<4>(vv, dd, (uint32_t)jj0, JSER, 1 + kind_attrs(ikind));
+                                z = vv[0];
+                                c0 = vv[1];
+                                c1 = vv[2];
+                                c2 = vv[3];
+                                have = true;
+                                key = k;
+                                row = (uint32_t)(P.ra + (int32_t)(k - P.rec_base));
+                                jc = jj0;
+                                j1 = jj1;
+                                ip0 = P.p0;
+                                ip1 = P.p1;
+                                ip2 = P.p2;
+                                itag = P.tag;
+                            }
                         }
                     }
-                    uint32_t li = ((uint32_t)((int32_t)xs + j0) - x0) * LD + lrow;
-                    for (int32_t q0 = j0; q0 < j1; q0 += CHUNK, li += CHUNK * LD) {
-                        const int32_t rem = j1 - q0;
+                    const unsigned long long busy = __ballot(have);
+                    if (busy == 0ull) {
+                        if (next >= nrec) break;
+                        continue;
+                    }
+                    // claim every cell of the remaining pixels
+                    int32_t cl0 = 0, cl1 = -1, cw1 = -1;
+                    if (have) {
+                        cl0 = (xs + jc - (int32_t)x0) / CELL_W;
+                        cl1 = (xs + j1 - 1 - (int32_t)x0) / CELL_W;
+                        const int32_t je = jc + CHUNK < j1 ? jc + CHUNK : j1;
+                        cw1 = (xs + je - 1 - (int32_t)x0) / CELL_W;
+                        for (int32_t c = cl0; c <= cl1; ++c) atomicMin(&s_cell[row * NCG + c], key);
+                    }
+                    bool win = have;
+                    for (int32_t c = cl0; c <= cw1; ++c) win = win && s_cell[row * NCG + c] == key;
+                    if (win) {
+                        const int32_t rem = j1 - jc;
+                        const uint32_t li = (uint32_t)(xs + jc - (int32_t)x0) * LD + row;
                         float zz[CHUNK], v0[CHUNK], v1[CHUNK], v2[CHUNK], cur[CHUNK];
+                        bool ps[CHUNK];
 #pragma unroll
                         for (int p = 0; p < CHUNK; ++p) {
                             zz[p] = z;
@@ -1108,22 +1173,74 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         if (has_depth) {
 #pragma unroll
                             for (int p = 0; p < CHUNK; ++p) cur[p] = s_dep[p < rem ? li + p * LD : li];
+                        } else {
+#pragma unroll
+                            for (int p = 0; p < CHUNK; ++p) cur[p] = 0.0f;
                         }
+                        uint32_t npass = 0;
 #pragma unroll
                         for (int p = 0; p < CHUNK; ++p) {
-                            if (p < rem && (!has_depth || depth_pass(F.depth_test, zz[p], cur[p]))) {
-                                const uint32_t a = li + p * LD;
-                                if (F.color_write) {
-                                    ++fo_acc;
-                                    uint32_t tg, py;
-                                    frag_payload(P, F.fmt, zz[p], v0[p], v1[p], v2[p], tg, py);
-                                    s_tag[a] = tg;
-                                    s_pay[a] = py;
+                            ps[p] = (p < rem) & depth_pass_u(dtest, zz[p], cur[p]);
+                            npass += ps[p] ? 1u : 0u;
+                        }
+                        if (F.color_write) {
+                            fo_acc += npass;
+                            // colour sources of the chunk's pixels (the shader kind is
+                            // decided once per chunk)
+                            uint32_t pay[CHUNK];
+                            uint32_t tg;
+                            if (ikind == KIND_FLAT) {
+                                tg = TAG_COLOR;
+#pragma unroll
+                                for (int p = 0; p < CHUNK; ++p) pay[p] = ip0;
+                            } else if (ikind == KIND_GOURAUD) {  // Color3f::to_color4 (color.rs:253-263)
+                                tg = TAG_COLOR;
+#pragma unroll
+                                for (int p = 0; p < CHUNK; ++p)
+                                    pay[p] = pack_pixel(F.fmt, as_u8(256.0f * v0[p]), as_u8(256.0f * v1[p]),
+                                                        as_u8(256.0f * v2[p]), 0xFF);
+                            } else {
+                                tg = itag;
+                                const float tw = (float)ip0, th = (float)ip1;
+                                if (ikind == KIND_TEX_CLAMP) {  // SamplerClamp (tex.rs:280-303)
+                                    const float mu = tw - 1.0f, mv = th - 1.0f;
+#pragma unroll
+                                    for (int p = 0; p < CHUNK; ++p) {
+                                        // Scanline::fragments z_div (raster.rs:61-68, vec.rs:666-676)
+                                        float su = (v0[p] / zz[p]) * tw, sv = (v1[p] / zz[p]) * th;
+                                        su = su < 0.0f ? 0.0f : su;
+                                        su = su > mu ? mu : su;
+                                        sv = sv < 0.0f ? 0.0f : sv;
+                                        sv = sv > mv ? mv : sv;
+                                        pay[p] = as_u32(floorf(sv)) * ip2 + as_u32(floorf(su));
+                                    }
+                                } else {  // SamplerRepeatPot (tex.rs:240-266)
+#pragma unroll
+                                    for (int p = 0; p < CHUNK; ++p) {
+                                        const uint32_t ui = (uint32_t)as_i32(floorf(tw * (v0[p] / zz[p]))) & (ip0 - 1);
+                                        const uint32_t vi = (uint32_t)as_i32(floorf(th * (v1[p] / zz[p]))) & (ip1 - 1);
+                                        pay[p] = vi * ip2 + ui;
+                                    }
                                 }
-                                if (has_depth && F.depth_write) s_dep[a] = zz[p];
+                            }
+#pragma unroll
+                            for (int p = 0; p < CHUNK; ++p) {
+                                if (ps[p]) {
+                                    s_tag[li + p * LD] = (uint16_t)tg;
+                                    s_pay[li + p * LD] = pay[p];
+                                }
                             }
                         }
+                        if (dwrite) {
+#pragma unroll
+                            for (int p = 0; p < CHUNK; ++p)
+                                if (ps[p]) s_dep[li + p * LD] = zz[p];
+                        }
+                        jc += CHUNK;
+                        if (jc >= j1) have = false;
                     }
+                    // release the claims (every claimant resets what it touched)
+                    for (int32_t c = cl0; c <= cl1; ++c) s_cell[row * NCG + c] = 0xFFFFFFFFu;
                 }
                 __syncthreads();
                 fq_head += nuse;
@@ -1138,8 +1255,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
             if (tg == TAG_COLOR) {
                 store_pixel(F, x0 + c, y0 + r, s_pay[li]);
             } else if (tg != TAG_NONE) {
-                const uint32_t t = ((const uint32_t*)B.draws[tg].tex)[s_pay[li]];
-                store_pixel(F, x0 + c, y0 + r, pack_pixel(F.fmt, t & 0xFF, (t >> 8) & 0xFF, (t >> 16) & 0xFF, t >> 24));
+                store_pixel(F, x0 + c, y0 + r, texel_pixel(B, F.fmt, dbase + tg, s_pay[li]));
             } else if (F.clear_color) {
                 store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
             }
