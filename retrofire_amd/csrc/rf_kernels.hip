@@ -903,13 +903,31 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
         const uint32_t nrows = min((uint32_t)TILE_H, F.height - y0);
         const uint32_t npx = ncols * nrows;
 
+        // 16-byte stores: whole tile row of 4-byte pixels, 16-byte aligned rows
+        const bool vec = ncols == (uint32_t)TW && F.fmt != PIX_RGB8 && ((uintptr_t)F.color & 15u) == 0 &&
+                         (F.color_stride & 15u) == 0 && (!has_depth || (((uintptr_t)F.depth & 15u) == 0 &&
+                                                                         (F.depth_stride & 3u) == 0));
         if (nruns == 0) {
             // Nothing drawn here: the fused clear is the only write.
-            for (uint32_t i = lane; i < npx; i += WAVE) {
-                const uint32_t r = i / ncols, c = i - r * ncols;
-                if (F.clear_color) store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
-                if (has_depth && F.clear_depth)
-                    F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = F.clear_depth_value;
+            if (vec) {
+                constexpr uint32_t QR = TW / 4;  // quads per row
+                const uint32_t cp = F.clear_pixel;
+                const float cz = F.clear_depth_value;
+                for (uint32_t q = lane; q < nrows * QR; q += WAVE) {
+                    const uint32_t r = q / QR, c = (q - r * QR) * 4;
+                    if (F.clear_color)
+                        *(uint4*)(F.color + (uint64_t)(y0 + r) * F.color_stride + 4ull * (x0 + c)) =
+                            make_uint4(cp, cp, cp, cp);
+                    if (has_depth && F.clear_depth)
+                        *(float4*)(F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 + c) = make_float4(cz, cz, cz, cz);
+                }
+            } else {
+                for (uint32_t i = lane; i < npx; i += WAVE) {
+                    const uint32_t r = i / ncols, c = i - r * ncols;
+                    if (F.clear_color) store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
+                    if (has_depth && F.clear_depth)
+                        F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = F.clear_depth_value;
+                }
             }
             continue;
         }
@@ -1248,18 +1266,58 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
         }
 
         // write-back: fetch the texels, colour and depth to the target
-        for (uint32_t i = lane; i < npx; i += WAVE) {
-            const uint32_t r = i / ncols, c = i - r * ncols;
-            const uint32_t li = c * LD + r;
-            const uint32_t tg = s_tag[li];
-            if (tg == TAG_COLOR) {
-                store_pixel(F, x0 + c, y0 + r, s_pay[li]);
-            } else if (tg != TAG_NONE) {
-                store_pixel(F, x0 + c, y0 + r, texel_pixel(B, F.fmt, dbase + tg, s_pay[li]));
-            } else if (F.clear_color) {
-                store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
+        if (vec && (F.clear_color || F.color_write)) {
+            // four pixels per lane; every load of a quad is issued before use
+            constexpr uint32_t QR = TW / 4;
+            for (uint32_t q = lane; q < nrows * QR; q += WAVE) {
+                const uint32_t r = q / QR, c = (q - r * QR) * 4;
+                uint8_t* crow = F.color + (uint64_t)(y0 + r) * F.color_stride + 4ull * (x0 + c);
+                uint32_t tg[4], py[4], px[4];
+                const uint32_t* tp[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    tg[k] = s_tag[(c + k) * LD + r];
+                    py[k] = s_pay[(c + k) * LD + r];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    tp[k] = tg[k] < TAG_COLOR ? (const uint32_t*)B.draws[dbase + tg[k]].tex : nullptr;
+                uint4 old = make_uint4(0, 0, 0, 0);
+                const bool keep = !F.clear_color && (tg[0] == TAG_NONE || tg[1] == TAG_NONE || tg[2] == TAG_NONE ||
+                                                     tg[3] == TAG_NONE);
+                if (keep) old = *(const uint4*)crow;
+                const uint32_t oldk[4] = {old.x, old.y, old.z, old.w};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (tg[k] == TAG_COLOR) {
+                        px[k] = py[k];
+                    } else if (tg[k] != TAG_NONE) {
+                        const uint32_t t = tp[k][py[k]];
+                        px[k] = pack_pixel(F.fmt, t & 0xFF, (t >> 8) & 0xFF, (t >> 16) & 0xFF, t >> 24);
+                    } else {
+                        px[k] = F.clear_color ? F.clear_pixel : oldk[k];
+                    }
+                }
+                *(uint4*)crow = make_uint4(px[0], px[1], px[2], px[3]);
+                if (has_depth)
+                    *(float4*)(F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 + c) =
+                        make_float4(s_dep[c * LD + r], s_dep[(c + 1) * LD + r], s_dep[(c + 2) * LD + r],
+                                    s_dep[(c + 3) * LD + r]);
             }
-            if (has_depth) F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = s_dep[li];
+        } else {
+            for (uint32_t i = lane; i < npx; i += WAVE) {
+                const uint32_t r = i / ncols, c = i - r * ncols;
+                const uint32_t li = c * LD + r;
+                const uint32_t tg = s_tag[li];
+                if (tg == TAG_COLOR) {
+                    store_pixel(F, x0 + c, y0 + r, s_pay[li]);
+                } else if (tg != TAG_NONE) {
+                    store_pixel(F, x0 + c, y0 + r, texel_pixel(B, F.fmt, dbase + tg, s_pay[li]));
+                } else if (F.clear_color) {
+                    store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
+                }
+                if (has_depth) F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = s_dep[li];
+            }
         }
         const unsigned long long fi_s = wave_sum_u64(fi_acc), fo_s = wave_sum_u64(fo_acc);
         if (lane == 0) {
