@@ -765,22 +765,22 @@ struct SlotP {
     uint32_t kind;
     uint32_t p0, p1, p2;   // flat: packed colour; textured: width, height, row stride (texels)
     uint32_t tag;          // draw - the tile's draw base
-    int32_t ra;            // first tile row with a span record
+    int32_t ra;            // first tile row with an edge record
     uint32_t rec_base;     // its record
-    uint32_t pad;
+    int32_t ysplit;        // first row of the lower trapezoid (absolute)
+    uint32_t rx0b, rx1b;   // pixel columns of the binned tiles (margin check)
+    uint32_t pad[2];
 };
-// Span record of one (triangle, tile row): the varyings at the span's first
-// pixel (after the x0 adjustment of ScanlineIter::next), its start and
-// fragment count.
-struct SpanRec {
-    float z, a0, a1, a2;
-    uint32_t xs_nfrag_h;   // xs | fragment count << 15 | trapezoid << 31
+// Edge record of one (triangle, tile row): the left and right edge state
+// after the serial row steps of ScanlineIter::next (raster.rs:84-113).
+struct EdgeRec {
+    float lx, rx, lz, l0, l1, l2;
 };
 
 #ifndef RF_REC_CAP
 #define RF_REC_CAP 384
 #endif
-constexpr int REC_CAP = RF_REC_CAP;  // span records per batch
+constexpr int REC_CAP = RF_REC_CAP;  // edge records per batch
 constexpr int FQ_CAP = 512;          // fan queue (ring); one run adds at most 64 * 7 fans
 constexpr int CELL_W = 8;            // pixel columns per ordering cell
 #ifndef RF_KMAX
@@ -873,7 +873,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
     __shared__ uint32_t s_pay[NPX];  // the pixel's colour source: payload
     __shared__ uint16_t s_tag[NPX];  //   and tag
     __shared__ SlotP s_slot[WAVE];
-    __shared__ SpanRec s_rec[REC_CAP];
+    __shared__ EdgeRec s_rec[REC_CAP];
     __shared__ uint32_t s_fq[FQ_CAP];  // g << 3 | fan
     __shared__ int8_t s_own[REC_CAP];  // slot whose records start here, or -1
     __shared__ uint32_t s_cell[TILE_H * NCG];
@@ -1047,6 +1047,9 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                     P.tag = di - dbase;
                     P.ra = ra;
                     P.rec_base = rec_base;
+                    P.ysplit = S.t[0].n ? (int32_t)S.t[0].y0i + (int32_t)S.t[0].n : -0x7FFFFFFF;
+                    P.rx0b = rx0b;
+                    P.rx1b = rx1b;
                     if (rows_in) s_own[rec_base] = (int8_t)lane;
                 }
                 // ---- edges: lane = triangle, walk its rows in the tile
@@ -1063,40 +1066,11 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         float ev[6] = {T.lx, T.rx, T.lz, T.la[0], T.la[1], T.la[2]};
                         const float ed[6] = {T.dlx, T.drx, T.dlz, T.dla[0], T.dla[1], T.dla[2]};
                         advance<6>(ev, ed, (uint32_t)(a - ty0), KMAX, 3 + na);
-                        float lx = ev[0], rx = ev[1], lz = ev[2], l0 = ev[3], l1 = ev[4], l2 = ev[5];
-                        for (int32_t y = a; y < b; ++y) {
-                            const float rx0 = round_up_to_half(lx), rx1 = round_up_to_half(rx);
-                            // v0.lerp(&v0.step(&dv_dx), x0 - v0.x)
-                            const float tw_ = rx0 - lx;
-                            SpanRec R;
-                            R.z = lz + ((lz + T.dvz) - lz) * tw_;
-                            R.a0 = l0 + ((l0 + T.dva[0]) - l0) * tw_;
-                            R.a1 = l1 + ((l1 + T.dva[1]) - l1) * tw_;
-                            R.a2 = l2 + ((l2 + T.dva[2]) - l2) * tw_;
-                            const uint32_t cnt = as_u32(rx1 - rx0);
-                            const uint64_t xs = as_usize(rx0);
-                            uint64_t xe = as_usize(rx1);
-                            if (xe < xs) xe = xs;
-                            uint32_t nfrag = 0, xsr = 0;
-                            if (xe > F.width || (uint32_t)y >= F.height) {
-                                errs |= ERRB_BOUNDS;
-                            } else {
-                                if (xe > xs && (xs < rx0b || xe > rx1b)) errs |= ERRB_MARGIN;
-                                // frags.i: the part of [xs, xe) inside this tile
-                                const uint64_t pa = xs > x0 ? xs : x0;
-                                const uint64_t pb = xe < (uint64_t)(x0 + TW) ? xe : (uint64_t)(x0 + TW);
-                                if (pb > pa) fi_acc += pb - pa;
-                                nfrag = (uint64_t)cnt < xe - xs ? cnt : (uint32_t)(xe - xs);
-                                xsr = (uint32_t)xs;
-                            }
-                            R.xs_nfrag_h = xsr | (nfrag << 15) | ((uint32_t)h << 31);
-                            s_rec[rec_base + (uint32_t)(y - (int32_t)y0 - ra)] = R;
-                            lx = lx + T.dlx;
-                            lz = lz + T.dlz;
-                            l0 = l0 + T.dla[0];
-                            l1 = l1 + T.dla[1];
-                            l2 = l2 + T.dla[2];
-                            rx = rx + T.drx;
+                        EdgeRec* out = &s_rec[rec_base + (uint32_t)(a - (int32_t)y0 - ra)];
+                        for (int32_t y = a; y < b; ++y, ++out) {
+                            *out = EdgeRec{ev[0], ev[1], ev[2], ev[3], ev[4], ev[5]};
+#pragma unroll
+                            for (int i = 0; i < 6; ++i) ev[i] = ev[i] + ed[i];
                         }
                     }
                 }
@@ -1121,21 +1095,45 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         carry = __shfl(f, WAVE - 1, WAVE);  // the scan is non-decreasing
                         next = next + cnt < nrec ? next + cnt : nrec;
                         if (valid) {
-                            const SpanRec R = s_rec[k];
+                            const EdgeRec R = s_rec[k];
                             const SlotP& P = s_slot[f];
-                            xs = (int32_t)(R.xs_nfrag_h & 0x7FFFu);
-                            const int32_t nfrag = (int32_t)((R.xs_nfrag_h >> 15) & 0xFFFFu);
-                            const uint32_t h = R.xs_nfrag_h >> 31;
-                            const int32_t jj0 = (int32_t)x0 - xs > 0 ? (int32_t)x0 - xs : 0;
-                            int32_t jj1 = (int32_t)x0 + TW - xs;
-                            if (jj1 > nfrag) jj1 = nfrag;
+                            const int32_t r = P.ra + (int32_t)(k - P.rec_base);
+                            const int32_t y = (int32_t)y0 + r;
+                            const int h = y < P.ysplit ? 0 : 1;
+                            ikind = P.kind;
+                            dz = P.dv[h][0];
+                            d0 = P.dv[h][1];
+                            d1 = P.dv[h][2];
+                            d2 = P.dv[h][3];
+                            // the span of this row (ScanlineIter::next, raster.rs:84-113)
+                            const float rx0 = round_up_to_half(R.lx), rx1 = round_up_to_half(R.rx);
+                            // v0.lerp(&v0.step(&dv_dx), x0 - v0.x)
+                            const float tw_ = rx0 - R.lx;
+                            float vv[4] = {R.lz + ((R.lz + dz) - R.lz) * tw_, R.l0 + ((R.l0 + d0) - R.l0) * tw_,
+                                           R.l1 + ((R.l1 + d1) - R.l1) * tw_, R.l2 + ((R.l2 + d2) - R.l2) * tw_};
+                            // 32-bit saturating casts: any value the 64-bit `as usize`
+                            // keeps apart from these is beyond the target and fails
+                            // the bounds check the same way
+                            const uint32_t cnt_f = as_u32(rx1 - rx0);
+                            const uint32_t xs32 = as_u32(rx0);
+                            uint32_t xe = as_u32(rx1);
+                            if (xe < xs32) xe = xs32;
+                            int32_t jj0 = 0, jj1 = 0;
+                            if (xe > F.width || (uint32_t)y >= F.height) {
+                                errs |= ERRB_BOUNDS;
+                            } else {
+                                if (xe > xs32 && (xs32 < P.rx0b || xe > P.rx1b)) errs |= ERRB_MARGIN;
+                                // frags.i: the part of [xs, xe) inside this tile
+                                const uint32_t pa = xs32 > x0 ? xs32 : x0;
+                                const uint32_t pb = xe < x0 + TW ? xe : x0 + TW;
+                                if (pb > pa) fi_acc += pb - pa;
+                                const uint32_t nfrag = cnt_f < xe - xs32 ? cnt_f : xe - xs32;
+                                xs = (int32_t)xs32;
+                                jj0 = (int32_t)x0 - xs > 0 ? (int32_t)x0 - xs : 0;
+                                jj1 = (int32_t)x0 + TW - xs;
+                                if (jj1 > (int32_t)nfrag) jj1 = (int32_t)nfrag;
+                            }
                             if (jj0 < jj1) {
-                                ikind = P.kind;
-                                dz = P.dv[h][0];
-                                d0 = P.dv[h][1];
-                                d1 = P.dv[h][2];
-                                d2 = P.dv[h][3];
-                                float vv[4] = {R.z, R.a0, R.a1, R.a2};
                                 const float dd[4] = {dz, d0, d1, d2};
                                 // the fragment iterator stepped jj0 times (Scanline::fragments)
                                 advance<4>(vv, dd, (uint32_t)jj0, JSER, 1 + kind_attrs(ikind));
@@ -1145,7 +1143,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                                 c2 = vv[3];
                                 have = true;
                                 key = k;
-                                row = (uint32_t)(P.ra + (int32_t)(k - P.rec_base));
+                                row = (uint32_t)r;
                                 jc = jj0;
                                 j1 = jj1;
                                 ip0 = P.p0;
