@@ -792,7 +792,7 @@ constexpr int CELL_W = 8;            // pixel columns per ordering cell
 constexpr int KMAX = RF_KMAX;  // row steps taken serially; more use the exact fast-forward
 constexpr int JSER = RF_JSER;  // pixel steps taken serially; more use the exact fast-forward
 #ifndef RF_REFILL_MIN
-#define RF_REFILL_MIN 1
+#define RF_REFILL_MIN 16
 #endif
 constexpr int REFILL_MIN = RF_REFILL_MIN;  // free lanes that trigger a refill while others still work
 
@@ -932,13 +932,14 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
             continue;
         }
 
-        // tile init: depth from the fused clear or the target; no fragment yet
-        for (uint32_t i = lane; i < npx; i += WAVE) {
-            const uint32_t r = i / ncols, c = i - r * ncols;
-            if (has_depth)
-                s_dep[c * LD + r] = F.clear_depth ? F.clear_depth_value
-                                                  : F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c];
-            s_tag[c * LD + r] = TAG_NONE;
+        // tile init (lane = row): depth from the fused clear or the target; no
+        // fragment yet
+        if (lane < nrows) {
+            const float* drow = has_depth ? F.depth + (uint64_t)(y0 + lane) * F.depth_stride + x0 : nullptr;
+            for (uint32_t c = 0; c < ncols; ++c) {
+                if (has_depth) s_dep[c * LD + lane] = F.clear_depth ? F.clear_depth_value : drow[c];
+                s_tag[c * LD + lane] = TAG_NONE;
+            }
         }
         __syncthreads();
 

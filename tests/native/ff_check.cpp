@@ -70,7 +70,29 @@ int main(int argc, char** argv) {
         const float s = (float)(U(rng) * 10 - 5), d = (float)((U(rng) - 0.5) * std::pow(10.0, -6 * U(rng)));
         for (uint32_t n = 0; n < 600; ++n) check(s, d, n);
     }
-    // 5. specials
+    // 5. sequences that arrive within a few steps of a binade boundary:
+    //    s starts J*m + o ulps below 2^(e+1) (or above 2^e, stepping down), so
+    //    the step that leaves the binade has an exact sum close to the edge
+    for (long i = 0; i < cases / 2; ++i) {
+        const int e = (int)(U(rng) * 30) - 10;
+        const float lo = std::ldexp(1.0f, e), hi = 2.0f * lo, u = std::ldexp(1.0f, e - 23);
+        const int m = 1 + (int)(U(rng) * 2000);
+        static const float fr[] = {0.0f, 0.5f, -0.5f, 0.25f, -0.25f, 0.375f, -0.375f, 0.125f};
+        const float d = ((float)m + fr[(int)(U(rng) * 8)]) * u;
+        const uint32_t J = 1 + (uint32_t)(U(rng) * 3000);
+        const float o = (float)(int)(U(rng) * 2 * m);
+        const float up = hi - ((float)J * (float)m + o) * u, dn = lo + ((float)J * (float)m + o) * u;
+        const uint32_t n = J + (uint32_t)(U(rng) * 12);
+        if (up >= lo) {
+            check(up, d, n);
+            check(-up, -d, n);
+        }
+        if (dn < hi) {
+            check(dn, -d, n);
+            check(-dn, d, n);
+        }
+    }
+    // 6. specials
     const float inf = INFINITY, nan = NAN;
     const float sp[] = {0.0f, -0.0f, 1e-40f, -1e-40f, 1e-38f, 3e38f, -3e38f, inf, -inf, nan, 1.0f, -1.0f};
     for (float s : sp)
