@@ -42,7 +42,40 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--calib-copy-mib", type=int, default=0,
+                   help="also run one device-to-device copy of this many MiB (scale reference for HBM counters)")
     return p.parse_args()
+
+
+def calib_copy(mib):
+    """One device-to-device hipMemcpy of `mib` MiB: a copy kernel whose HBM
+    traffic is known (read mib, write mib), the reference that turns the
+    FETCH_SIZE / WRITE_SIZE counters of a profiler run into bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc:
+        raise RuntimeError(f"hipMemcpy failed ({rc})")
+
+
+def measured_traffic(tile_w):
+    """HBM bytes per k_raster launch from the committed counter summary, if it
+    was taken for this tile width (profiles/k_raster_traffic.json)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "k_raster_traffic.json")) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get("bytes_per_launch") if t.get("tile_w") == tile_w else None
 
 
 def camera_frames(n, w, h, seed=12345):
@@ -131,7 +164,7 @@ def main():
         "stage_ms": {n: round(x, 4) for n, x in zip(names, prof)},
         "roofline": {"bound": "hbm", "kernel": "k_raster", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic(batch.tile_w),
                      "alg_bytes_per_launch": alg_bytes, "launch_ms": round(raster_ms, 4)},
         "pipeline_alg_GBs": round(alg_frame_bytes * a.frames / (total_prof_ms * 1e-3) / 1e9, 1),
     }
@@ -163,6 +196,8 @@ def main():
     batch.free()
     for fb in fbs:
         fb.free()
+    if a.calib_copy_mib > 0:
+        calib_copy(a.calib_copy_mib)
     print(json.dumps(out), flush=True)
 
 
