@@ -131,6 +131,10 @@ rf_status rf_device_close(rf_device* dev);
 const char* rf_status_str(rf_status s);
 /* Bytes of device memory the library holds for scratch and residency. */
 uint64_t rf_device_bytes(const rf_device* dev);
+/* Rasterizer tile width (32 or 64 pixel columns; 0 = the default, 32) used by
+ * rf_render on this device. No counterpart in the reference (a tuning knob of
+ * this implementation); results are identical for every width. */
+rf_status rf_device_set_tile_width(rf_device* dev, uint32_t tile_w);
 
 /* ---- plain path: the drop-in for render() ------------------------------
  * Synchronous, like the reference call: uploads the draw's data and the

@@ -43,7 +43,7 @@ struct Device {
     uint64_t bytes = 0;  // device memory held
     Scratch plain;
     DevBuf up_pos, up_attr, up_tris, up_tex, up_color, up_depth, up_desc;
-    uint32_t plain_tile_w = 64;
+    uint32_t plain_tile_w = 32;  // rasterizer tile width of rf_render
     uint32_t plain_extra_cap = 0, plain_runs_cap = 0;
 };
 

@@ -16,6 +16,15 @@ from test_oracle_kats import GOLDEN, hello_tri_inputs, read_ppm, textured_quad_i
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=[32, 64], ids=["tw32", "tw64"])
+def tile_width(request):
+    """Every parity case runs with both rasterizer tile widths."""
+    dev = rf.default_device()
+    dev.set_tile_width(request.param)
+    yield request.param
+    dev.set_tile_width(0)
+
+
 def test_textured_quad_golden_gpu():  # core/tests/rendering.rs:18-60
     verts, uvs, faces, tex, mvp, vp = textured_quad_inputs()
     fb = np.zeros((256, 256, 3), np.uint8)
