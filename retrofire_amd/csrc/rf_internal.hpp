@@ -15,9 +15,9 @@ namespace rf {
 
 constexpr uint32_t WAVE_SIZE = 64;
 constexpr uint32_t MAX_DIM = 16384;         // largest target side (span records pack x in 16 bits)
-constexpr uint32_t TILE_ROWS = 64;          // rasterizer tile height (one lane per row)
+constexpr uint32_t TILE_ROWS = TILE_HEIGHT;  // rasterizer tile height
 #ifndef RF_RASTER_WG_PER_CU
-#define RF_RASTER_WG_PER_CU 4
+#define RF_RASTER_WG_PER_CU (TILE_HEIGHT > 32 ? 4 : 8)
 #endif
 constexpr uint32_t RASTER_WG_PER_CU = RF_RASTER_WG_PER_CU;  // persistent rasterizer workgroups per CU
 

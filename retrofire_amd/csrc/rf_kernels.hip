@@ -30,7 +30,7 @@
 namespace rf {
 
 constexpr int WAVE = 64;
-constexpr int TILE_H = 64;  // one lane per tile row
+constexpr int TILE_H = (int)TILE_HEIGHT;  // tile rows
 constexpr int BIN_MARGIN = 2;  // extra pixels around a triangle's bounding box
 // Tuning constants (overridable at build time for sweeps).
 #ifndef RF_FF_MIN
@@ -759,17 +759,15 @@ __device__ __forceinline__ uint32_t find_frame_tile(const FrameDesc* f, uint32_t
     return lo;
 }
 
-// Per-triangle data of a batch slot.
+// Per-triangle data of a batch slot (64 B).
 struct SlotP {
     float dv[2][4];        // d(z, a0, a1, a2)/dx of the upper and lower trapezoid
-    uint32_t kind;
+    uint32_t kind_tag;     // shader kind | (draw - the tile's draw base) << 16
     uint32_t p0, p1, p2;   // flat: packed colour; textured: width, height, row stride (texels)
-    uint32_t tag;          // draw - the tile's draw base
-    int32_t ra;            // first tile row with an edge record
-    uint32_t rec_base;     // its record
+    uint32_t ra_rec;       // first tile row with an edge record | that record << 16
     int32_t ysplit;        // first row of the lower trapezoid (absolute)
-    uint32_t rx0b, rx1b;   // pixel columns of the binned tiles (margin check)
-    uint32_t pad[2];
+    uint32_t rxb;          // pixel columns of the binned tiles, first | end << 16 (margin check)
+    uint32_t pad;
 };
 // Edge record of one (triangle, tile row): the left and right edge state
 // after the serial row steps of ScanlineIter::next (raster.rs:84-113).
@@ -778,7 +776,7 @@ struct EdgeRec {
 };
 
 #ifndef RF_REC_CAP
-#define RF_REC_CAP 384
+#define RF_REC_CAP (TILE_H > 32 ? 384 : 128)
 #endif
 constexpr int REC_CAP = RF_REC_CAP;  // edge records per batch
 constexpr int FQ_CAP = 512;          // fan queue (ring); one run adds at most 64 * 7 fans
@@ -1035,7 +1033,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         P.dv[h][0] = S.t[h].dvz;
                         for (int c = 0; c < 3; ++c) P.dv[h][1 + c] = S.t[h].dva[c];
                     }
-                    P.kind = kind;
+                    P.kind_tag = kind | ((di - dbase) << 16);
                     const DrawDesc& D = B.draws[di];
                     if (kind == KIND_FLAT) {
                         const uint32_t fc = D.flat_rgba;
@@ -1045,12 +1043,9 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                     }
                     P.p1 = D.tex_h;
                     P.p2 = D.tex_stride;
-                    P.tag = di - dbase;
-                    P.ra = ra;
-                    P.rec_base = rec_base;
+                    P.ra_rec = (uint32_t)ra | (rec_base << 16);
                     P.ysplit = S.t[0].n ? (int32_t)S.t[0].y0i + (int32_t)S.t[0].n : -0x7FFFFFFF;
-                    P.rx0b = rx0b;
-                    P.rx1b = rx1b;
+                    P.rxb = rx0b | (rx1b << 16);
                     if (rows_in) s_own[rec_base] = (int8_t)lane;
                 }
                 // ---- edges: lane = triangle, walk its rows in the tile
@@ -1098,10 +1093,10 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         if (valid) {
                             const EdgeRec R = s_rec[k];
                             const SlotP& P = s_slot[f];
-                            const int32_t r = P.ra + (int32_t)(k - P.rec_base);
+                            const int32_t r = (int32_t)(P.ra_rec & 0xFFFFu) + (int32_t)(k - (P.ra_rec >> 16));
                             const int32_t y = (int32_t)y0 + r;
                             const int h = y < P.ysplit ? 0 : 1;
-                            ikind = P.kind;
+                            ikind = P.kind_tag & 0xFFFFu;
                             dz = P.dv[h][0];
                             d0 = P.dv[h][1];
                             d1 = P.dv[h][2];
@@ -1123,7 +1118,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                             if (xe > F.width || (uint32_t)y >= F.height) {
                                 errs |= ERRB_BOUNDS;
                             } else {
-                                if (xe > xs32 && (xs32 < P.rx0b || xe > P.rx1b)) errs |= ERRB_MARGIN;
+                                if (xe > xs32 && (xs32 < (P.rxb & 0xFFFFu) || xe > (P.rxb >> 16))) errs |= ERRB_MARGIN;
                                 // frags.i: the part of [xs, xe) inside this tile
                                 const uint32_t pa = xs32 > x0 ? xs32 : x0;
                                 const uint32_t pb = xe < x0 + TW ? xe : x0 + TW;
@@ -1150,7 +1145,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                                 ip0 = P.p0;
                                 ip1 = P.p1;
                                 ip2 = P.p2;
-                                itag = P.tag;
+                                itag = P.kind_tag >> 16;
                             }
                         }
                     }

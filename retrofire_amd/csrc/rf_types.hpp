@@ -14,6 +14,13 @@
 
 namespace rf {
 
+#ifndef RF_TILE_H
+#define RF_TILE_H 32
+#endif
+// Rasterizer tile height in rows (the binning grid and the rasterizer agree on it).
+constexpr uint32_t TILE_HEIGHT = RF_TILE_H;
+
+
 // Shader kinds (the built-in vertex/fragment shader pairs).
 enum ShaderKind : uint32_t {
     KIND_FLAT = 0,            // constant Color4, varying ()
