@@ -420,33 +420,47 @@ __global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
 }
 
 // ---------------------------------------------------------------------------
-// k_scan: exclusive scan of run counts (one workgroup of 1024 threads)
+// k_scan_*: exclusive scan of the run counts over all tiles, in three steps
+// (blocks of 1024 tiles; the block totals; their offsets added back)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_scan(BatchDesc B) {
-    __shared__ uint32_t warp_tot[16];
+__global__ __launch_bounds__(1024) void k_scan_local(BatchDesc B) {
+    __shared__ uint32_t wave_tot[16];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
+    const uint32_t i = blockIdx.x * 1024 + tid;
+    const uint32_t v = i < B.n_tiles ? B.run_count[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = wave_excl_scan(v, &tot);
+    if (lane == 0) wave_tot[w] = tot;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < w; ++k) off += wave_tot[k];
+    if (i < B.n_tiles) {
+        B.run_off[i] = off + ex;
+        B.run_cursor[i] = 0;
+    }
+    if (tid == 1023) B.scan_blk[blockIdx.x] = off + ex + v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(BatchDesc B) {
+    __shared__ uint32_t wave_tot[16];
     __shared__ uint32_t carry;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
+    const uint32_t nb = (B.n_tiles + 1023) / 1024;
     if (tid == 0) carry = 0;
     __syncthreads();
-    for (uint32_t base = 0; base < B.n_tiles; base += 1024) {
+    for (uint32_t base = 0; base < nb; base += 1024) {
         const uint32_t i = base + tid;
-        const uint32_t v = i < B.n_tiles ? B.run_count[i] : 0u;
+        const uint32_t v = i < nb ? B.scan_blk[i] : 0u;
         uint32_t tot;
         const uint32_t ex = wave_excl_scan(v, &tot);
-        if (lane == 0) warp_tot[w] = tot;
+        if (lane == 0) wave_tot[w] = tot;
         __syncthreads();
         uint32_t off = carry;
-        for (uint32_t k = 0; k < w; ++k) off += warp_tot[k];
-        if (i < B.n_tiles) {
-            B.run_off[i] = off + ex;
-            B.run_cursor[i] = 0;
-        }
+        for (uint32_t k = 0; k < w; ++k) off += wave_tot[k];
+        if (i < nb) B.scan_blk[i] = off + ex;
         __syncthreads();
-        if (tid == 0) {
-            uint32_t s = 0;
-            for (uint32_t k = 0; k < 16; ++k) s += warp_tot[k];
-            carry += s;
-        }
+        if (tid == 0)
+            for (uint32_t k = 0; k < 16; ++k) carry += wave_tot[k];
         __syncthreads();
     }
     if (tid == 0) {
@@ -456,21 +470,23 @@ __global__ __launch_bounds__(1024) void k_scan(BatchDesc B) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// k_bin: one run per (group, tile)
-// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_scan_add(BatchDesc B) {
+    const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+    if (i < B.n_tiles && blockIdx.x > 0) B.run_off[i] += B.scan_blk[blockIdx.x];
+}
+
 __global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
     const uint32_t lane = lane_id();
     const uint32_t group = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (group >= B.n_groups) return;
     const uint64_t g = (uint64_t)group * WAVE + lane;
-    bool has = false;
+    bool valid = false;
     uint32_t r0 = 0, r1 = 0, base = 0, ntx = 0;
     if (g < B.n_tris) {
         r0 = B.rect[2 * g];
         r1 = B.rect[2 * g + 1];
-        has = r0 != 0xFFFFFFFFu;
-        if (has) {
+        valid = r0 != 0xFFFFFFFFu;
+        if (valid) {
             const uint32_t di = find_draw_tri(B.draws, B.n_draws, g);
             const FrameDesc& F = B.frames[B.draws[di].frame];
             base = F.tile_base;
@@ -478,98 +494,148 @@ __global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
         }
     }
     const uint32_t cap = B.runs_cap;
-    for_each_tile(has, r0, r1, base, ntx, [&](uint32_t tile, unsigned long long m) {
-        const uint32_t pos = B.run_off[tile] + atomicAdd(&B.run_cursor[tile], 1u);
-        if (pos < cap) {
-            Run r;
-            r.group = group;
-            r.pad = 0;
-            r.mask = m;
-            B.runs[pos] = r;
-        }
-    });
-}
-
-// ---------------------------------------------------------------------------
-// k_binsort: sort each tile's runs by group (bitonic, ascending)
-// ---------------------------------------------------------------------------
-constexpr int SORT_LDS = 4096;
-
-__global__ __launch_bounds__(256) void k_binsort(BatchDesc B) {
-    __shared__ unsigned long long key[SORT_LDS];
-    const uint32_t tile = blockIdx.x;
-    if (tile >= B.n_tiles) return;
-    const uint32_t off = B.run_off[tile];
-    uint32_t n = B.run_off[tile + 1] - off;
-    if (off >= B.runs_cap) return;
-    if (off + n > B.runs_cap) n = B.runs_cap - off;
-    if (n <= 1) return;
-    Run* runs = B.runs + off;
-    const uint32_t tid = threadIdx.x, nt = blockDim.x;
-    // Is it already sorted? (common: one group per tile, or groups landing in order)
-    __shared__ int unsorted;
-    if (tid == 0) unsorted = 0;
-    __syncthreads();
-    for (uint32_t i = tid; i + 1 < n; i += nt)
-        if (runs[i].group > runs[i + 1].group) unsorted = 1;
-    __syncthreads();
-    if (!unsorted) return;
-    uint32_t p2 = 1;
-    while (p2 < n) p2 <<= 1;
-    // Bitonic network in its all-ascending form: the first step of each merge
-    // compares i with i ^ (k - 1), later steps i with i ^ j, always putting
-    // the smaller key at the lower index. Entries at indices >= n act as +inf
-    // keys, so any pair whose upper index is >= n is a no-op and is skipped.
-    if (p2 <= SORT_LDS) {
-        // key = group << 32 | original index
-        for (uint32_t i = tid; i < n; i += nt) key[i] = ((unsigned long long)runs[i].group << 32) | i;
-        __syncthreads();
-        for (uint32_t k = 2; k <= p2; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                const uint32_t x = (j == (k >> 1)) ? (k - 1) : j;
-                for (uint32_t i = tid; i < n; i += nt) {
-                    const uint32_t l = i ^ x;
-                    if (l > i && l < n) {
-                        const unsigned long long a = key[i], b = key[l];
-                        if (a > b) {
-                            key[i] = b;
-                            key[l] = a;
-                        }
-                    }
-                }
-                __syncthreads();
+    // The tiles of the group in increasing order (as in for_each_tile); lane k
+    // collects the k-th (tile, mask), and every 64 of them take their run slots
+    // with one round of atomics.
+    const uint32_t tx0 = r0 & 0xFFFF, tx1 = r1 & 0xFFFF, ty1 = r1 >> 16;
+    uint32_t cx = tx0, cy = r0 >> 16;
+    uint32_t my_tile = 0, k = 0;
+    unsigned long long my_mask = 0;
+    auto flush = [&]() {
+        if (lane < k) {
+            const uint32_t pos = B.run_off[my_tile] + atomicAdd(&B.run_cursor[my_tile], 1u);
+            if (pos < cap) {
+                Run r;
+                r.group = group;
+                r.pad = 0;
+                r.mask = my_mask;
+                B.runs[pos] = r;
             }
         }
-        // gather masks in sorted order (read all first, then write)
-        unsigned long long m[SORT_LDS / 256];
-        uint32_t cnt = 0;
-        for (uint32_t i = tid; i < n; i += nt) m[cnt++] = runs[(uint32_t)key[i]].mask;
-        __syncthreads();
-        cnt = 0;
-        for (uint32_t i = tid; i < n; i += nt) {
-            Run r;
-            r.group = (uint32_t)(key[i] >> 32);
-            r.pad = 0;
-            r.mask = m[cnt++];
-            runs[i] = r;
+        k = 0;
+    };
+    while (true) {
+        const uint32_t next = valid ? base + cy * ntx + cx : 0xFFFFFFFFu;
+        const uint32_t t = wave_min_u32(next);
+        if (t == 0xFFFFFFFFu) break;
+        const bool mine = valid && next == t;
+        const unsigned long long m = __ballot(mine);
+        if (lane == k) {
+            my_tile = t;
+            my_mask = m;
         }
-    } else {
-        // Large bin: the same network in place on the run records in global memory.
-        for (uint32_t k = 2; k <= p2; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                const uint32_t x = (j == (k >> 1)) ? (k - 1) : j;
-                for (uint32_t i = tid; i < n; i += nt) {
-                    const uint32_t l = i ^ x;
-                    if (l > i && l < n) {
-                        const Run a = runs[i], b = runs[l];
-                        if (a.group > b.group) {
-                            runs[i] = b;
-                            runs[l] = a;
+        if (++k == (uint32_t)WAVE) flush();
+        if (mine && ++cx > tx1) {
+            cx = tx0;
+            if (++cy > ty1) valid = false;
+        }
+    }
+    flush();
+}
+constexpr int SORT_LDS = 1024;  // bins up to this many runs sort in LDS
+
+// Sorts the runs held one per lane (lanes >= n hold none) by group: a bitonic
+// network over the 64 lanes; lane i ends with the i-th run. Groups are
+// distinct within a tile.
+__device__ __forceinline__ void wave_sort_runs(uint32_t n, uint32_t& group, unsigned long long& mask) {
+    const uint32_t lane = lane_id();
+    const bool in = lane < n;
+    const uint32_t g = in ? group : 0xFFFFFFFFu;
+    const uint32_t nextg = __shfl_down(g, 1, WAVE);
+    if (!__ballot(in && lane + 1 < n && g > nextg)) return;  // already in order
+    unsigned long long kk = ((unsigned long long)g << 6) | lane;
+    for (uint32_t ksz = 2; ksz <= (uint32_t)WAVE; ksz <<= 1) {
+        for (uint32_t js = ksz >> 1; js > 0; js >>= 1) {
+            const unsigned long long other = __shfl_xor(kk, js, WAVE);
+            const bool take_min = ((lane & js) == 0) == ((lane & ksz) == 0);
+            kk = take_min ? (other < kk ? other : kk) : (other > kk ? other : kk);
+        }
+    }
+    mask = __shfl(mask, (uint32_t)(kk & 63u), WAVE);
+    group = (uint32_t)(kk >> 6);
+}
+
+// Orders the runs of each tile with more than 64 of them by group (k_bin
+// appends them in arrival order; the rasterizer sorts lists of up to 64 runs
+// itself, in registers). One wave per 64 tiles: bins of up to SORT_LDS runs
+// sort in LDS, larger ones in place in global memory.
+__global__ __launch_bounds__(64) void k_binsort(BatchDesc B) {
+    __shared__ unsigned long long key[SORT_LDS];
+    __shared__ unsigned long long msk[SORT_LDS];
+    const uint32_t lane = lane_id();
+    const uint32_t t0 = blockIdx.x * WAVE;
+    const uint32_t t = t0 + lane;
+    uint32_t off = 0, n = 0;
+    if (t < B.n_tiles) {
+        off = B.run_off[t];
+        n = B.run_off[t + 1] - off;
+        if (off >= B.runs_cap) n = 0;
+        else if (off + n > B.runs_cap) n = B.runs_cap - off;
+    }
+    unsigned long long todo = __ballot(n > (uint32_t)WAVE);
+    while (todo) {
+        const uint32_t j = (uint32_t)(__ffsll((long long)todo) - 1);
+        todo &= todo - 1;
+        const uint32_t o = __shfl(off, j, WAVE), nn = __shfl(n, j, WAVE);
+        Run* runs = B.runs + o;
+        if (nn <= (uint32_t)SORT_LDS) {
+            bool unsorted = false;
+            for (uint32_t i = lane; i + 1 < nn; i += WAVE) unsorted |= runs[i].group > runs[i + 1].group;
+            if (!__ballot(unsorted)) continue;
+            uint32_t p2 = 1;
+            while (p2 < nn) p2 <<= 1;
+            for (uint32_t i = lane; i < nn; i += WAVE) {
+                key[i] = ((unsigned long long)runs[i].group << 32) | i;
+                msk[i] = runs[i].mask;
+            }
+            __syncthreads();
+            // all-ascending bitonic network: the first step of each merge
+            // compares i with i ^ (k - 1), later steps i with i ^ j; indices
+            // >= nn act as +inf keys
+            for (uint32_t ksz = 2; ksz <= p2; ksz <<= 1) {
+                for (uint32_t js = ksz >> 1; js > 0; js >>= 1) {
+                    const uint32_t x = (js == (ksz >> 1)) ? (ksz - 1) : js;
+                    for (uint32_t i = lane; i < nn; i += WAVE) {
+                        const uint32_t l = i ^ x;
+                        if (l > i && l < nn) {
+                            const unsigned long long a = key[i], c = key[l];
+                            if (a > c) {
+                                key[i] = c;
+                                key[l] = a;
+                            }
                         }
                     }
+                    __syncthreads();
                 }
-                __threadfence_block();
-                __syncthreads();
+            }
+            for (uint32_t i = lane; i < nn; i += WAVE) {
+                Run w;
+                w.group = (uint32_t)(key[i] >> 32);
+                w.pad = 0;
+                w.mask = msk[(uint32_t)key[i]];
+                runs[i] = w;
+            }
+            __syncthreads();
+        } else {
+            // very large bin: the same network in place on the run records
+            uint32_t p2 = 1;
+            while (p2 < nn) p2 <<= 1;
+            for (uint32_t ksz = 2; ksz <= p2; ksz <<= 1) {
+                for (uint32_t js = ksz >> 1; js > 0; js >>= 1) {
+                    const uint32_t x = (js == (ksz >> 1)) ? (ksz - 1) : js;
+                    for (uint32_t i = lane; i < nn; i += WAVE) {
+                        const uint32_t l = i ^ x;
+                        if (l > i && l < nn) {
+                            const Run a = runs[i], c = runs[l];
+                            if (a.group > c.group) {
+                                runs[i] = c;
+                                runs[l] = a;
+                            }
+                        }
+                    }
+                    __threadfence_block();
+                    __syncthreads();
+                }
             }
         }
     }
@@ -947,10 +1013,28 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
         uint32_t dbase = 0;                 // draw of the tile's first triangle
         bool dbase_set = false;
 
+        // the tile's runs in group order: lists of up to 64 are loaded one per
+        // lane and sorted here; longer ones were sorted by k_binsort
+        uint32_t lrun_g = 0;
+        unsigned long long lrun_m = 0;
+        if (nruns <= (uint32_t)WAVE) {
+            if (lane < nruns) {
+                const Run r = B.runs[off + lane];
+                lrun_g = r.group;
+                lrun_m = r.mask;
+            }
+            wave_sort_runs(nruns, lrun_g, lrun_m);
+        }
         for (uint32_t ri = 0; ri <= nruns; ++ri) {
             // 1. append the next run's triangles (their fans) to the queue
             if (ri < nruns) {
-                const Run run = B.runs[off + ri];
+                Run run;
+                if (nruns <= (uint32_t)WAVE) {
+                    run.group = __shfl(lrun_g, ri, WAVE);
+                    run.mask = __shfl(lrun_m, ri, WAVE);
+                } else {
+                    run = B.runs[off + ri];
+                }
                 const uint64_t g = (uint64_t)run.group * WAVE + lane;
                 const bool has = (run.mask >> lane) & 1ull;
                 const uint32_t nf = has ? (B.hdr[g] & 7u) : 0u;
@@ -1357,11 +1441,14 @@ static hipError_t launch_impl(const BatchDesc& B, hipStream_t s, uint32_t raster
     mark(1);
     if (B.n_groups) hipLaunchKernelGGL(k_setup, dim3((B.n_groups + 3) / 4), dim3(256), 0, s, B);
     mark(2);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B);
+    const uint32_t nb = (B.n_tiles + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_local, dim3(nb ? nb : 1), dim3(1024), 0, s, B);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, B);
+    if (nb > 1) hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, s, B);
     mark(3);
     if (B.n_groups) hipLaunchKernelGGL(k_bin, dim3((B.n_groups + 3) / 4), dim3(256), 0, s, B);
     mark(4);
-    hipLaunchKernelGGL(k_binsort, dim3(B.n_tiles), dim3(256), 0, s, B);
+    hipLaunchKernelGGL(k_binsort, dim3((B.n_tiles + WAVE - 1) / WAVE), dim3(WAVE), 0, s, B);
     mark(5);
     if (B.tile_w == 32)
         hipLaunchKernelGGL(k_raster<32>, dim3(raster_grid), dim3(64), 0, s, B);

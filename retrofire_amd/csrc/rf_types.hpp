@@ -133,6 +133,7 @@ struct BatchDesc {
     uint32_t* run_count; // per tile
     uint32_t* run_off;   // per tile + 1 (exclusive scan)
     uint32_t* run_cursor;
+    uint32_t* scan_blk;  // per 1024 tiles: run total, then its offset
     Run* runs;
     uint32_t runs_cap;
     uint32_t* tile_counter;  // persistent raster work counter

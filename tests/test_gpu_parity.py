@@ -202,3 +202,23 @@ def test_tall_thin_fan():
     mvp = M.then(M.translate3(0, 0, 2.0), M.perspective(1.0, np.float32(w) / np.float32(h), 0.1, 10.0))
     o, g = run_both(1, tris, pos, cols, mvp, M.viewport(0, h, w, 0), w, h, ctx=rf.Context(clear=True, face_cull=None))
     assert_same(o, g)
+
+
+@pytest.mark.parametrize("n", [3000, 30000, 80000])
+def test_many_groups_in_one_tile(n):
+    """Many 64-triangle groups touching the same few tiles: the per-tile run
+    lists (one run per group) are ordered in registers (<= 64 runs), in LDS
+    (<= 1024) and in place in global memory (more); any misordering shows as
+    a different winner of overlapping fragments."""
+    rng = np.random.default_rng(n)
+    w, h = 64, 48
+    c = rng.uniform([20, 14, 0.2], [44, 34, 0.9], size=(n, 1, 3)).astype(np.float32)
+    pos = (c + rng.uniform([-6, -6, -0.05], [6, 6, 0.05], size=(n, 3, 3)).astype(np.float32)).reshape(-1, 3)
+    # screen-space coordinates through an orthographic mvp: x, y in pixels, z in (0, 1)
+    mvp = M.then(M.scale3(2.0 / w, 2.0 / h, 1.0), M.translate3(-1.0, -1.0, 0.0))
+    vp = M.viewport(0, h, w, 0)
+    tris = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    cols = rng.uniform(0, 1, size=(3 * n, 3)).astype(np.float32)
+    o, g = run_both(1, tris, pos, cols, mvp, vp, w, h, ctx=rf.Context(clear=True, face_cull=None))
+    assert_same(o, g)
+    assert g[2][6] > 500
