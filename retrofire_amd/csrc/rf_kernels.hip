@@ -47,12 +47,36 @@ constexpr int CHUNK = RF_CHUNK;    // pixels per lane processed together in a sp
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Cross-lane steps through DPP (row shifts, mirrors and row broadcasts: VALU
+// operations, no LDS round trip). Every caller runs with all 64 lanes active.
+template <int CTRL, int ROW_MASK = 0xF, int BANK_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, BANK_MASK, false);
+}
+constexpr int DPP_QUAD_1032 = 0xB1, DPP_QUAD_2301 = 0x4E, DPP_ROW_MIRROR = 0x140, DPP_ROW_HALF_MIRROR = 0x141,
+              DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR3 = 0x113, DPP_ROW_SHR4 = 0x114,
+              DPP_ROW_SHR8 = 0x118, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        uint32_t w = __shfl_xor(v, o, WAVE);
-        v = w < v ? w : v;
-    }
-    return v;
+    v = min(v, dpp_u32<DPP_QUAD_1032>(v, v));
+    v = min(v, dpp_u32<DPP_QUAD_2301>(v, v));
+    v = min(v, dpp_u32<DPP_ROW_HALF_MIRROR>(v, v));
+    v = min(v, dpp_u32<DPP_ROW_MIRROR>(v, v));
+    v = min(v, dpp_u32<DPP_ROW_BCAST15, 0xA>(v, v));
+    v = min(v, dpp_u32<DPP_ROW_BCAST31, 0xC>(v, v));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+// Inclusive scan with operator op over the lanes; `id` is op's identity.
+template <class Op>
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t id, Op op) {
+    uint32_t x = op(v, dpp_u32<DPP_ROW_SHR1>(id, v));
+    x = op(x, dpp_u32<DPP_ROW_SHR2>(id, v));
+    x = op(x, dpp_u32<DPP_ROW_SHR3>(id, v));
+    x = op(x, dpp_u32<DPP_ROW_SHR4, 0xF, 0xE>(id, x));
+    x = op(x, dpp_u32<DPP_ROW_SHR8, 0xF, 0xC>(id, x));
+    x = op(x, dpp_u32<DPP_ROW_BCAST15, 0xA>(id, x));
+    x = op(x, dpp_u32<DPP_ROW_BCAST31, 0xC>(id, x));
+    return x;
 }
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
@@ -60,13 +84,8 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 // Exclusive prefix sum over the wave.
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) {
-    const uint32_t lane = lane_id();
-    uint32_t x = v;
-    for (int o = 1; o < WAVE; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, WAVE);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    *total = __shfl(x, WAVE - 1, WAVE);
+    const uint32_t x = wave_incl_scan(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+    *total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
     return x - v;
 }
 
@@ -914,12 +933,9 @@ __device__ __forceinline__ void advance(float (&v)[K], const float (&d)[K], uint
 }
 
 __device__ __forceinline__ int32_t wave_incl_max_i32(int32_t v) {
-    const uint32_t lane = lane_id();
-    for (int o = 1; o < WAVE; o <<= 1) {
-        const int32_t y = __shfl_up(v, o, WAVE);
-        if (lane >= (uint32_t)o) v = y > v ? y : v;
-    }
-    return v;
+    // v >= -1: order-preserving shift to unsigned
+    const uint32_t x = wave_incl_scan((uint32_t)(v + 1), 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+    return (int32_t)x - 1;
 }
 
 // The depth test of Context::depth_test (ctx.rs:86-89) without branches.
@@ -1172,6 +1188,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         const bool valid = !have && k < nrec;
                         int32_t f = wave_incl_max_i32(valid ? (int32_t)s_own[k] : -1);
                         f = f > carry ? f : carry;
+                        f = f < 0 ? 0 : f;  // every valid record has a slot at or before it
                         carry = __shfl(f, WAVE - 1, WAVE);  // the scan is non-decreasing
                         next = next + cnt < nrec ? next + cnt : nrec;
                         if (valid) {
