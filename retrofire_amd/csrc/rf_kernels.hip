@@ -226,13 +226,13 @@ __device__ int clip_polygon(CV* poly, int n, CV* tmp) {
 }
 
 __device__ __forceinline__ void write_scr(ScrTri* dst, const float* s0, const float* s1,
-                                          const float* s2, uint32_t draw) {
+                                          const float* s2, uint32_t draw, uint32_t kind) {
     float* d = &dst->v[0][0];
     for (int c = 0; c < 6; ++c) d[c] = s0[c];
     for (int c = 0; c < 6; ++c) d[6 + c] = s1[c];
     for (int c = 0; c < 6; ++c) d[12 + c] = s2[c];
     dst->draw = draw;
-    dst->pad = 0;
+    dst->kind = kind;
 }
 
 struct RectAcc {
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
                     v2[c] = B.sv[6 * s2 + c];
                 }
                 if (!culled(F.face_cull, backface(v0, v1, v2))) {
-                    write_scr(&B.scr[g], v0, v1, v2, di);
+                    write_scr(&B.scr[g], v0, v1, v2, di, D->kind);
                     ra.add(v0);
                     ra.add(v1);
                     ra.add(v2);
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
                     for (int k = 1; k + 1 < n && j < cnt; ++k) {
                         if (!(keep & (1u << k))) continue;
                         ScrTri* dst = (j == 0) ? &B.scr[g] : &B.extra[extra_base + j - 1];
-                        write_scr(dst, sc[0], sc[k], sc[k + 1], di);
+                        write_scr(dst, sc[0], sc[k], sc[k + 1], di, D->kind);
                         ra.add(sc[0]);
                         ra.add(sc[k]);
                         ra.add(sc[k + 1]);
@@ -438,6 +438,15 @@ __global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
                   [&](uint32_t tile, unsigned long long) { atomicAdd(&B.run_count[tile], 1u); });
 }
 
+__device__ __forceinline__ uint32_t find_frame_tile(const FrameDesc* f, uint32_t n, uint32_t t) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (f[mid].tile_base <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------------
 // k_scan_*: exclusive scan of the run counts over all tiles, in three steps
 // (blocks of 1024 tiles; the block totals; their offsets added back)
@@ -456,6 +465,7 @@ __global__ __launch_bounds__(1024) void k_scan_local(BatchDesc B) {
     if (i < B.n_tiles) {
         B.run_off[i] = off + ex;
         B.run_cursor[i] = 0;
+        B.tile_frame[i] = find_frame_tile(B.frames, B.n_frames, i);
     }
     if (tid == 1023) B.scan_blk[blockIdx.x] = off + ex + v;
 }
@@ -835,14 +845,6 @@ __device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint
     *(uint32_t*)(p + 4ull * x) = v;
 }
 
-__device__ __forceinline__ uint32_t find_frame_tile(const FrameDesc* f, uint32_t n, uint32_t t) {
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (f[mid].tile_base <= t) lo = mid; else hi = mid;
-    }
-    return lo;
-}
 
 // Per-triangle data of a batch slot (64 B).
 struct SlotP {
@@ -866,6 +868,10 @@ struct EdgeRec {
 constexpr int REC_CAP = RF_REC_CAP;  // edge records per batch
 constexpr int FQ_CAP = 512;          // fan queue (ring); one run adds at most 64 * 7 fans
 constexpr int CELL_W = 8;            // pixel columns per ordering cell
+#ifndef RF_TILE_GRAB
+#define RF_TILE_GRAB 2
+#endif
+constexpr int TILE_GRAB = RF_TILE_GRAB;  // tiles a wave takes from the shared counter at once
 #ifndef RF_KMAX
 #define RF_KMAX 160
 #endif
@@ -957,17 +963,20 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
     __shared__ uint32_t s_fq[FQ_CAP];  // g << 3 | fan
     __shared__ int8_t s_own[REC_CAP];  // slot whose records start here, or -1
     __shared__ uint32_t s_cell[TILE_H * NCG];
-    __shared__ uint32_t s_tile;
 
     const uint32_t lane = lane_id();
     for (uint32_t i = lane; i < (uint32_t)(TILE_H * NCG); i += WAVE) s_cell[i] = 0xFFFFFFFFu;
+    uint32_t tcur = 0, tend = 0;  // tiles taken from the shared counter, TILE_GRAB at a time
     for (;;) {
-        if (lane == 0) s_tile = atomicAdd(B.tile_counter, 1u);
-        __syncthreads();
-        const uint32_t tile = s_tile;
-        __syncthreads();
+        if (tcur == tend) {
+            uint32_t t0 = 0;
+            if (lane == 0) t0 = atomicAdd(B.tile_counter, (uint32_t)TILE_GRAB);
+            tcur = (uint32_t)__builtin_amdgcn_readfirstlane((int)t0);
+            tend = tcur + TILE_GRAB;
+        }
+        const uint32_t tile = tcur++;
         if (tile >= B.n_tiles) break;
-        const uint32_t fi = find_frame_tile(B.frames, B.n_frames, tile);
+        const uint32_t fi = B.tile_frame[tile];
         const FrameDesc F = B.frames[fi];
         const uint32_t lt = tile - F.tile_base;
         const uint32_t tx = lt % F.ntx, ty = lt / F.ntx;
@@ -1084,7 +1093,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         vc[c] = st->v[2][c];
                     }
                     di = st->draw;
-                    kind = B.draws[di].kind;
+                    kind = st->kind;
                     na = kind_attrs(kind);
                     tri_setup(S, va, vb, vc, na);
                     const uint32_t rr0 = B.rect[2 * gg], rr1 = B.rect[2 * gg + 1];

@@ -99,7 +99,7 @@ struct FrameStats {
 struct ScrTri {
     float v[3][6];
     uint32_t draw;
-    uint32_t pad;
+    uint32_t kind;   // the draw's shader kind
 };
 
 // A bin run: the triangles of group `group` (64 consecutive global triangle
@@ -134,6 +134,7 @@ struct BatchDesc {
     uint32_t* run_off;   // per tile + 1 (exclusive scan)
     uint32_t* run_cursor;
     uint32_t* scan_blk;  // per 1024 tiles: run total, then its offset
+    uint32_t* tile_frame;  // per tile: its frame
     Run* runs;
     uint32_t runs_cap;
     uint32_t* tile_counter;  // persistent raster work counter
