@@ -130,7 +130,7 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
     B.tile_frame = (uint32_t*)tile_frame.p;
     B.runs = (Run*)runs.p;
     B.extra_count = (uint32_t*)misc.p;
-    B.tile_counter = (uint32_t*)misc.p + 1;
+    B.tile_counter = (uint32_t*)misc.p + 16;  // TILE_PARTS words
     B.err = (uint32_t*)misc.p + 2;
     B.stats = (FrameStats*)stats.p;
     return hipSuccess;

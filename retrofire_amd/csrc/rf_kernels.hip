@@ -495,7 +495,7 @@ __global__ __launch_bounds__(1024) void k_scan_top(BatchDesc B) {
     if (tid == 0) {
         B.run_off[B.n_tiles] = carry;
         if (carry > B.runs_cap) atomicOr(B.err, ERRB_RUNS);
-        *B.tile_counter = 0;
+        for (uint32_t k = 0; k < TILE_PARTS; ++k) B.tile_counter[k] = 0;
     }
 }
 
@@ -872,6 +872,10 @@ constexpr int CELL_W = 8;            // pixel columns per ordering cell
 #define RF_TILE_GRAB 2
 #endif
 constexpr int TILE_GRAB = RF_TILE_GRAB;  // tiles a wave takes from the shared counter at once
+#ifndef RF_TILE_TAIL
+#define RF_TILE_TAIL 512
+#endif
+constexpr int TILE_TAIL = RF_TILE_TAIL;  // the last tiles of a partition are taken one at a time
 #ifndef RF_KMAX
 #define RF_KMAX 160
 #endif
@@ -966,16 +970,35 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
 
     const uint32_t lane = lane_id();
     for (uint32_t i = lane; i < (uint32_t)(TILE_H * NCG); i += WAVE) s_cell[i] = 0xFFFFFFFFu;
-    uint32_t tcur = 0, tend = 0;  // tiles taken from the shared counter, TILE_GRAB at a time
+    // Tiles come from TILE_PARTS contiguous partitions with a counter each.
+    // Workgroups are dispatched round-robin over the XCDs, so a workgroup
+    // starts on partition blockIdx.x % TILE_PARTS: neighbouring tiles (which
+    // share triangles) meet in one XCD's L2. A wave takes TILE_GRAB tiles per
+    // atomic and moves on to the next partition when its own is done.
+    uint32_t part = blockIdx.x % TILE_PARTS, parts_left = TILE_PARTS;
+    uint32_t tcur = 0, tend = 0;
+    bool tail_near = false;
     for (;;) {
-        if (tcur == tend) {
+        while (tcur >= tend && parts_left) {
+            const uint32_t pb = (uint32_t)(((uint64_t)B.n_tiles * part) / TILE_PARTS);
+            const uint32_t pe = (uint32_t)(((uint64_t)B.n_tiles * (part + 1)) / TILE_PARTS);
+            // near a partition's end single tiles keep the tail short
+            const uint32_t grab = tail_near ? 1u : (uint32_t)TILE_GRAB;
             uint32_t t0 = 0;
-            if (lane == 0) t0 = atomicAdd(B.tile_counter, (uint32_t)TILE_GRAB);
-            tcur = (uint32_t)__builtin_amdgcn_readfirstlane((int)t0);
-            tend = tcur + TILE_GRAB;
+            if (lane == 0) t0 = atomicAdd(&B.tile_counter[part], grab);
+            t0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)t0);
+            if (t0 < pe - pb) {
+                tcur = pb + t0;
+                tend = min(tcur + grab, pe);
+                tail_near = t0 + (uint32_t)TILE_TAIL >= pe - pb;
+            } else {
+                part = (part + 1) % TILE_PARTS;
+                --parts_left;
+                tail_near = true;  // other partitions are in their tails too
+            }
         }
+        if (tcur >= tend) break;
         const uint32_t tile = tcur++;
-        if (tile >= B.n_tiles) break;
         const uint32_t fi = B.tile_frame[tile];
         const FrameDesc F = B.frames[fi];
         const uint32_t lt = tile - F.tile_base;
@@ -1448,7 +1471,7 @@ __global__ void k_reset(BatchDesc B) {
     if (i == 0) {
         *B.extra_count = 0;
         *B.err = 0;
-        *B.tile_counter = 0;
+        for (uint32_t k = 0; k < TILE_PARTS; ++k) B.tile_counter[k] = 0;
     }
 }
 

@@ -19,6 +19,9 @@ namespace rf {
 #endif
 // Rasterizer tile height in rows (the binning grid and the rasterizer agree on it).
 constexpr uint32_t TILE_HEIGHT = RF_TILE_H;
+// The rasterizer's tiles are split into this many contiguous partitions, one
+// per XCD (accelerator complex die, each with its own L2) of the MI355X.
+constexpr uint32_t TILE_PARTS = 8;
 
 
 // Shader kinds (the built-in vertex/fragment shader pairs).
@@ -137,7 +140,7 @@ struct BatchDesc {
     uint32_t* tile_frame;  // per tile: its frame
     Run* runs;
     uint32_t runs_cap;
-    uint32_t* tile_counter;  // persistent raster work counter
+    uint32_t* tile_counter;  // persistent raster work counters, one per tile partition (TILE_PARTS)
     uint32_t* err;
 };
 
