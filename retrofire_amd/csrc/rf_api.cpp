@@ -111,7 +111,8 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
     if ((e = run_count.ensure(ntl * sizeof(uint32_t), counter))) return e;
     if ((e = run_off.ensure((ntl + 1) * sizeof(uint32_t), counter))) return e;
     if ((e = run_cursor.ensure(ntl * sizeof(uint32_t), counter))) return e;
-    if ((e = scan_blk.ensure((ntl / 1024 + 2) * sizeof(uint32_t), counter))) return e;
+    if ((e = scan_blk.ensure(2 * (ntl / 1024 + 2) * sizeof(uint32_t), counter))) return e;
+    if ((e = tile_list.ensure(ntl * sizeof(uint32_t), counter))) return e;
     if ((e = tile_frame.ensure(ntl * sizeof(uint32_t), counter))) return e;
     if ((e = runs.ensure(std::max<uint64_t>(B.runs_cap, 1) * sizeof(Run), counter))) return e;
     if ((e = misc.ensure(256, counter))) return e;
@@ -128,9 +129,11 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
     B.run_cursor = (uint32_t*)run_cursor.p;
     B.scan_blk = (uint32_t*)scan_blk.p;
     B.tile_frame = (uint32_t*)tile_frame.p;
+    B.tile_list = (uint32_t*)tile_list.p;
+    B.busy_count = (uint32_t*)misc.p + 8;
     B.runs = (Run*)runs.p;
     B.extra_count = (uint32_t*)misc.p;
-    B.tile_counter = (uint32_t*)misc.p + 16;  // TILE_PARTS words
+    B.tile_counter = (uint32_t*)misc.p + 16;  // TILE_PARTS + 1 words (busy partitions, empty tiles)
     B.err = (uint32_t*)misc.p + 2;
     B.stats = (FrameStats*)stats.p;
     return hipSuccess;
@@ -138,7 +141,7 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
 
 void Scratch::release(uint64_t* counter) {
     for (DevBuf* b : {&cv, &ca, &sv, &hdr, &scr, &extra, &rect, &run_count, &run_off, &run_cursor,
-                      &scan_blk, &tile_frame, &runs, &misc, &stats})
+                      &scan_blk, &tile_frame, &tile_list, &runs, &misc, &stats})
         b->release(counter);
 }
 

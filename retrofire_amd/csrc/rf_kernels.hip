@@ -451,57 +451,86 @@ __device__ __forceinline__ uint32_t find_frame_tile(const FrameDesc* f, uint32_t
 // k_scan_*: exclusive scan of the run counts over all tiles, in three steps
 // (blocks of 1024 tiles; the block totals; their offsets added back)
 // ---------------------------------------------------------------------------
+// Two scans side by side: run counts (offsets of the tiles' run lists) and
+// "has runs" flags: B.tile_list lists the tiles with runs first, then the
+// others (which the rasterizer only clears).
 __global__ __launch_bounds__(1024) void k_scan_local(BatchDesc B) {
-    __shared__ uint32_t wave_tot[16];
+    __shared__ uint32_t wave_tot[2][16];
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
     const uint32_t i = blockIdx.x * 1024 + tid;
     const uint32_t v = i < B.n_tiles ? B.run_count[i] : 0u;
-    uint32_t tot;
+    const uint32_t f = v != 0 ? 1u : 0u;
+    uint32_t tot, ftot;
     const uint32_t ex = wave_excl_scan(v, &tot);
-    if (lane == 0) wave_tot[w] = tot;
+    const uint32_t fex = wave_excl_scan(f, &ftot);
+    if (lane == 0) {
+        wave_tot[0][w] = tot;
+        wave_tot[1][w] = ftot;
+    }
     __syncthreads();
-    uint32_t off = 0;
-    for (uint32_t k = 0; k < w; ++k) off += wave_tot[k];
+    uint32_t off = 0, foff = 0;
+    for (uint32_t k = 0; k < w; ++k) {
+        off += wave_tot[0][k];
+        foff += wave_tot[1][k];
+    }
     if (i < B.n_tiles) {
         B.run_off[i] = off + ex;
-        B.run_cursor[i] = 0;
+        B.run_cursor[i] = foff + fex;  // block-local list position until k_scan_add
         B.tile_frame[i] = find_frame_tile(B.frames, B.n_frames, i);
     }
-    if (tid == 1023) B.scan_blk[blockIdx.x] = off + ex + v;
+    if (tid == 1023) {
+        B.scan_blk[2 * blockIdx.x] = off + ex + v;
+        B.scan_blk[2 * blockIdx.x + 1] = foff + fex + f;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_scan_top(BatchDesc B) {
-    __shared__ uint32_t wave_tot[16];
-    __shared__ uint32_t carry;
+    __shared__ uint32_t wave_tot[2][16];
+    __shared__ uint32_t carry[2];
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
     const uint32_t nb = (B.n_tiles + 1023) / 1024;
-    if (tid == 0) carry = 0;
+    if (tid < 2) carry[tid] = 0;
     __syncthreads();
     for (uint32_t base = 0; base < nb; base += 1024) {
         const uint32_t i = base + tid;
-        const uint32_t v = i < nb ? B.scan_blk[i] : 0u;
-        uint32_t tot;
+        const uint32_t v = i < nb ? B.scan_blk[2 * i] : 0u, f = i < nb ? B.scan_blk[2 * i + 1] : 0u;
+        uint32_t tot, ftot;
         const uint32_t ex = wave_excl_scan(v, &tot);
-        if (lane == 0) wave_tot[w] = tot;
+        const uint32_t fex = wave_excl_scan(f, &ftot);
+        if (lane == 0) {
+            wave_tot[0][w] = tot;
+            wave_tot[1][w] = ftot;
+        }
         __syncthreads();
-        uint32_t off = carry;
-        for (uint32_t k = 0; k < w; ++k) off += wave_tot[k];
-        if (i < nb) B.scan_blk[i] = off + ex;
+        uint32_t off = carry[0], foff = carry[1];
+        for (uint32_t k = 0; k < w; ++k) {
+            off += wave_tot[0][k];
+            foff += wave_tot[1][k];
+        }
+        if (i < nb) {
+            B.scan_blk[2 * i] = off + ex;
+            B.scan_blk[2 * i + 1] = foff + fex;
+        }
         __syncthreads();
-        if (tid == 0)
-            for (uint32_t k = 0; k < 16; ++k) carry += wave_tot[k];
+        if (tid < 2)
+            for (uint32_t k = 0; k < 16; ++k) carry[tid] += wave_tot[tid][k];
         __syncthreads();
     }
     if (tid == 0) {
-        B.run_off[B.n_tiles] = carry;
-        if (carry > B.runs_cap) atomicOr(B.err, ERRB_RUNS);
-        for (uint32_t k = 0; k < TILE_PARTS; ++k) B.tile_counter[k] = 0;
+        B.run_off[B.n_tiles] = carry[0];
+        *B.busy_count = carry[1];
+        if (carry[0] > B.runs_cap) atomicOr(B.err, ERRB_RUNS);
+        for (uint32_t k = 0; k <= TILE_PARTS; ++k) B.tile_counter[k] = 0;
     }
 }
 
 __global__ __launch_bounds__(1024) void k_scan_add(BatchDesc B) {
     const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
-    if (i < B.n_tiles && blockIdx.x > 0) B.run_off[i] += B.scan_blk[blockIdx.x];
+    if (i >= B.n_tiles) return;
+    if (blockIdx.x > 0) B.run_off[i] += B.scan_blk[2 * blockIdx.x];
+    const uint32_t busy_before = B.run_cursor[i] + B.scan_blk[2 * blockIdx.x + 1];
+    B.tile_list[B.run_count[i] != 0 ? busy_before : *B.busy_count + (i - busy_before)] = i;
+    B.run_cursor[i] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
@@ -954,6 +983,44 @@ __device__ __forceinline__ bool depth_pass_u(int32_t dt, float nz, float cur) {
            ((dt == DT_GREATER) & (cur > nz));
 }
 
+#ifndef RF_EMPTY_GRAB
+#define RF_EMPTY_GRAB 16
+#endif
+constexpr int EMPTY_GRAB = RF_EMPTY_GRAB;  // tiles without runs a wave takes at once
+
+// The fused clear (Frame::clear) of one tile that nothing draws into: 16-byte
+// stores where whole rows of 4-byte pixels are aligned.
+template <int TW>
+__device__ __forceinline__ void clear_tile(const FrameDesc& F, uint32_t tile, uint32_t lane) {
+    const bool has_depth = F.depth != nullptr;
+    if (!F.clear_color && !(has_depth && F.clear_depth)) return;
+    const uint32_t lt = tile - F.tile_base;
+    const uint32_t x0 = (lt % F.ntx) * TW, y0 = (lt / F.ntx) * TILE_H;
+    const uint32_t ncols = min((uint32_t)TW, F.width - x0), nrows = min((uint32_t)TILE_H, F.height - y0);
+    const bool vec = ncols == (uint32_t)TW && F.fmt != PIX_RGB8 && ((uintptr_t)F.color & 15u) == 0 &&
+                     (F.color_stride & 15u) == 0 &&
+                     (!has_depth || (((uintptr_t)F.depth & 15u) == 0 && (F.depth_stride & 3u) == 0));
+    if (vec) {
+        constexpr uint32_t QR = TW / 4;  // quads per row
+        const uint32_t cp = F.clear_pixel;
+        const float cz = F.clear_depth_value;
+        for (uint32_t q = lane; q < nrows * QR; q += WAVE) {
+            const uint32_t r = q / QR, c = (q - r * QR) * 4;
+            if (F.clear_color)
+                *(uint4*)(F.color + (uint64_t)(y0 + r) * F.color_stride + 4ull * (x0 + c)) = make_uint4(cp, cp, cp, cp);
+            if (has_depth && F.clear_depth)
+                *(float4*)(F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 + c) = make_float4(cz, cz, cz, cz);
+        }
+    } else {
+        for (uint32_t i = lane; i < ncols * nrows; i += WAVE) {
+            const uint32_t r = i / ncols, c = i - r * ncols;
+            if (F.clear_color) store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
+            if (has_depth && F.clear_depth)
+                F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = F.clear_depth_value;
+        }
+    }
+}
+
 template <int TW>
 __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
     constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
@@ -975,13 +1042,14 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
     // starts on partition blockIdx.x % TILE_PARTS: neighbouring tiles (which
     // share triangles) meet in one XCD's L2. A wave takes TILE_GRAB tiles per
     // atomic and moves on to the next partition when its own is done.
+    const uint32_t n_busy = *B.busy_count;  // tiles with runs, listed in B.tile_list
     uint32_t part = blockIdx.x % TILE_PARTS, parts_left = TILE_PARTS;
     uint32_t tcur = 0, tend = 0;
     bool tail_near = false;
     for (;;) {
         while (tcur >= tend && parts_left) {
-            const uint32_t pb = (uint32_t)(((uint64_t)B.n_tiles * part) / TILE_PARTS);
-            const uint32_t pe = (uint32_t)(((uint64_t)B.n_tiles * (part + 1)) / TILE_PARTS);
+            const uint32_t pb = (uint32_t)(((uint64_t)n_busy * part) / TILE_PARTS);
+            const uint32_t pe = (uint32_t)(((uint64_t)n_busy * (part + 1)) / TILE_PARTS);
             // near a partition's end single tiles keep the tail short
             const uint32_t grab = tail_near ? 1u : (uint32_t)TILE_GRAB;
             uint32_t t0 = 0;
@@ -998,7 +1066,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
             }
         }
         if (tcur >= tend) break;
-        const uint32_t tile = tcur++;
+        const uint32_t tile = B.tile_list[tcur++];
         const uint32_t fi = B.tile_frame[tile];
         const FrameDesc F = B.frames[fi];
         const uint32_t lt = tile - F.tile_base;
@@ -1020,27 +1088,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                          (F.color_stride & 15u) == 0 && (!has_depth || (((uintptr_t)F.depth & 15u) == 0 &&
                                                                          (F.depth_stride & 3u) == 0));
         if (nruns == 0) {
-            // Nothing drawn here: the fused clear is the only write.
-            if (vec) {
-                constexpr uint32_t QR = TW / 4;  // quads per row
-                const uint32_t cp = F.clear_pixel;
-                const float cz = F.clear_depth_value;
-                for (uint32_t q = lane; q < nrows * QR; q += WAVE) {
-                    const uint32_t r = q / QR, c = (q - r * QR) * 4;
-                    if (F.clear_color)
-                        *(uint4*)(F.color + (uint64_t)(y0 + r) * F.color_stride + 4ull * (x0 + c)) =
-                            make_uint4(cp, cp, cp, cp);
-                    if (has_depth && F.clear_depth)
-                        *(float4*)(F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 + c) = make_float4(cz, cz, cz, cz);
-                }
-            } else {
-                for (uint32_t i = lane; i < npx; i += WAVE) {
-                    const uint32_t r = i / ncols, c = i - r * ncols;
-                    if (F.clear_color) store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
-                    if (has_depth && F.clear_depth)
-                        F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = F.clear_depth_value;
-                }
-            }
+            clear_tile<TW>(F, tile, lane);
             continue;
         }
 
@@ -1456,6 +1504,34 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
         if (lane == 0 && e) atomicOr(B.err, e);
         __syncthreads();
     }
+
+    // Then the tiles without runs, EMPTY_GRAB per atomic (waves that finish
+    // the busy tiles early fill the tail with them): their fused clear is the
+    // only write.
+    const uint32_t n_empty = B.n_tiles - n_busy;
+    uint32_t ecur_fi = 0xFFFFFFFFu;
+    FrameDesc E;
+    for (;;) {
+        uint32_t e0 = 0;
+        if (lane == 0) e0 = atomicAdd(&B.tile_counter[TILE_PARTS], (uint32_t)EMPTY_GRAB);
+        e0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)e0);
+        if (e0 >= n_empty) break;
+        const uint32_t ne = min((uint32_t)EMPTY_GRAB, n_empty - e0);
+        uint32_t my_tile = 0, my_fi = 0;
+        if (lane < ne) {
+            my_tile = B.tile_list[n_busy + e0 + lane];
+            my_fi = B.tile_frame[my_tile];
+        }
+        for (uint32_t j = 0; j < ne; ++j) {
+            const uint32_t tile = (uint32_t)__builtin_amdgcn_readlane((int)my_tile, (int)j);
+            const uint32_t fi = (uint32_t)__builtin_amdgcn_readlane((int)my_fi, (int)j);
+            if (fi != ecur_fi) {
+                E = B.frames[fi];
+                ecur_fi = fi;
+            }
+            clear_tile<TW>(E, tile, lane);
+        }
+    }
 }
 
 // Zeroes per-batch counters (a kernel, so it can live inside a captured graph).
@@ -1493,7 +1569,7 @@ static hipError_t launch_impl(const BatchDesc& B, hipStream_t s, uint32_t raster
     const uint32_t nb = (B.n_tiles + 1023) / 1024;
     hipLaunchKernelGGL(k_scan_local, dim3(nb ? nb : 1), dim3(1024), 0, s, B);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, B);
-    if (nb > 1) hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, s, B);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb ? nb : 1), dim3(1024), 0, s, B);
     mark(3);
     if (B.n_groups) hipLaunchKernelGGL(k_bin, dim3((B.n_groups + 3) / 4), dim3(256), 0, s, B);
     mark(4);

@@ -138,6 +138,8 @@ struct BatchDesc {
     uint32_t* run_cursor;
     uint32_t* scan_blk;  // per 1024 tiles: run total, then its offset
     uint32_t* tile_frame;  // per tile: its frame
+    uint32_t* tile_list;   // tiles with runs, in tile order
+    uint32_t* busy_count;  // their number
     Run* runs;
     uint32_t runs_cap;
     uint32_t* tile_counter;  // persistent raster work counters, one per tile partition (TILE_PARTS)
