@@ -898,7 +898,7 @@ constexpr int REC_CAP = RF_REC_CAP;  // edge records per batch
 constexpr int FQ_CAP = 512;          // fan queue (ring); one run adds at most 64 * 7 fans
 constexpr int CELL_W = 8;            // pixel columns per ordering cell
 #ifndef RF_TILE_GRAB
-#define RF_TILE_GRAB 2
+#define RF_TILE_GRAB 1
 #endif
 constexpr int TILE_GRAB = RF_TILE_GRAB;  // tiles a wave takes from the shared counter at once
 #ifndef RF_TILE_TAIL
