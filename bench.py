@@ -100,6 +100,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import retrofire_amd as rf
+    from retrofire_amd import _ffi
     from retrofire_amd import batch as RB
     from retrofire_amd import scenes as S
 
@@ -156,7 +157,7 @@ def main():
         "data": "synthetic (seeded crates scene, resident in HBM)",
         "config": {"workload": "crates1080: 1000 textured crates (24k verts, 12k tris, 256x256 RGBA8, "
                                "SamplerRepeatPot), 1920x1080 RGBA8 + f32 depth, clear per frame",
-                   "frames_per_step": a.frames, "tile_w": batch.tile_w, "tile_h": 64,
+                   "frames_per_step": a.frames, "tile_w": batch.tile_w, "tile_h": int(_ffi.lib().rf_tile_height()),
                    "parallelism": "single GPU"},
         "mfrag_per_s": round(mfrag, 1),
         "frags_i_per_frame": frags_i // a.frames,

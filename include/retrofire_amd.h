@@ -135,6 +135,9 @@ uint64_t rf_device_bytes(const rf_device* dev);
  * rf_render on this device. No counterpart in the reference (a tuning knob of
  * this implementation); results are identical for every width. */
 rf_status rf_device_set_tile_width(rf_device* dev, uint32_t tile_w);
+/* Rasterizer tile height (rows) this library was built with. No counterpart
+ * in the reference (an implementation parameter, reported by benchmarks). */
+uint32_t rf_tile_height(void);
 
 /* ---- plain path: the drop-in for render() ------------------------------
  * Synchronous, like the reference call: uploads the draw's data and the

@@ -71,6 +71,7 @@ SIGNATURES = {
     "rf_status_str": (C.c_char_p, [_st]),
     "rf_device_bytes": (C.c_uint64, [_vp]),
     "rf_device_set_tile_width": (_st, [_vp, C.c_uint32]),
+    "rf_tile_height": (C.c_uint32, []),
     "rf_render": (_st, [_vp, C.POINTER(RfDraw), C.POINTER(RfTarget), C.POINTER(RfCtx), C.POINTER(RfStats)]),
     "rf_mesh_create": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.c_uint64,
                              C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(_vp)]),

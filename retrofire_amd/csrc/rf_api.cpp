@@ -255,6 +255,8 @@ uint64_t rf_device_bytes(const rf_device* dev) {
     return dev ? reinterpret_cast<const Device*>(dev)->bytes : 0;
 }
 
+uint32_t rf_tile_height(void) { return TILE_ROWS; }
+
 rf_status rf_device_set_tile_width(rf_device* dev, uint32_t tile_w) {
     if (!dev || (tile_w != 0 && tile_w != 32 && tile_w != 64)) return RF_E_ARG;
     reinterpret_cast<Device*>(dev)->plain_tile_w = tile_w ? tile_w : 32;
