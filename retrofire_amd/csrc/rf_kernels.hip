@@ -26,6 +26,7 @@
 #include "rf_ff.hpp"
 #include "rf_math.hpp"
 #include "rf_types.hpp"
+#include "rf_internal.hpp"
 
 namespace rf {
 
@@ -1021,6 +1022,36 @@ __device__ __forceinline__ void clear_tile(const FrameDesc& F, uint32_t tile, ui
     }
 }
 
+// Clears the tiles without runs (listed after the others in B.tile_list):
+// one wave per EMPTY_GRAB list entries.
+__global__ __launch_bounds__(256) void k_clear_empty(BatchDesc B) {
+    const uint32_t lane = lane_id();
+    const uint32_t w = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const uint32_t n_busy = *B.busy_count, n_empty = B.n_tiles - n_busy;
+    const uint32_t e0 = w * EMPTY_GRAB;
+    if (e0 >= n_empty) return;
+    const uint32_t ne = min((uint32_t)EMPTY_GRAB, n_empty - e0);
+    uint32_t my_tile = 0, my_fi = 0;
+    if (lane < ne) {
+        my_tile = B.tile_list[n_busy + e0 + lane];
+        my_fi = B.tile_frame[my_tile];
+    }
+    uint32_t cur_fi = 0xFFFFFFFFu;
+    FrameDesc F;
+    for (uint32_t j = 0; j < ne; ++j) {
+        const uint32_t tile = (uint32_t)__builtin_amdgcn_readlane((int)my_tile, (int)j);
+        const uint32_t fi = (uint32_t)__builtin_amdgcn_readlane((int)my_fi, (int)j);
+        if (fi != cur_fi) {
+            F = B.frames[fi];
+            cur_fi = fi;
+        }
+        if (B.tile_w == 32)
+            clear_tile<32>(F, tile, lane);
+        else
+            clear_tile<64>(F, tile, lane);
+    }
+}
+
 template <int TW>
 __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
     constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
@@ -1508,7 +1539,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
     // Then the tiles without runs, EMPTY_GRAB per atomic (waves that finish
     // the busy tiles early fill the tail with them): their fused clear is the
     // only write.
-    const uint32_t n_empty = B.n_tiles - n_busy;
+    const uint32_t n_empty = B.clear_in_raster ? B.n_tiles - n_busy : 0u;
     uint32_t ecur_fi = 0xFFFFFFFFu;
     FrameDesc E;
     for (;;) {
@@ -1554,10 +1585,17 @@ __global__ void k_reset(BatchDesc B) {
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-static hipError_t launch_impl(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev) {
+static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev,
+                              const SideStream* side) {
+    BatchDesc B = B0;
+    B.clear_in_raster = side ? 0u : 1u;
     const uint32_t reset_n = B.n_tiles > B.n_frames ? B.n_tiles : B.n_frames;
     auto mark = [&](int i) {
         if (ev) (void)hipEventRecord(ev[i], s);
+    };
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) {
+        if (r != hipSuccess && e == hipSuccess) e = r;
     };
     mark(0);
     hipLaunchKernelGGL(k_reset, dim3((reset_n + 255) / 256 + 1), dim3(256), 0, s, B);
@@ -1570,6 +1608,14 @@ static hipError_t launch_impl(const BatchDesc& B, hipStream_t s, uint32_t raster
     hipLaunchKernelGGL(k_scan_local, dim3(nb ? nb : 1), dim3(1024), 0, s, B);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, B);
     hipLaunchKernelGGL(k_scan_add, dim3(nb ? nb : 1), dim3(1024), 0, s, B);
+    if (side) {
+        // the tiles without runs are known now: clear them beside the rest
+        chk(hipEventRecord(side->fork, s));
+        chk(hipStreamWaitEvent(side->s, side->fork, 0));
+        hipLaunchKernelGGL(k_clear_empty, dim3((B.n_tiles + 4 * EMPTY_GRAB - 1) / (4 * EMPTY_GRAB)), dim3(256), 0,
+                           side->s, B);
+        chk(hipEventRecord(side->join, side->s));
+    }
     mark(3);
     if (B.n_groups) hipLaunchKernelGGL(k_bin, dim3((B.n_groups + 3) / 4), dim3(256), 0, s, B);
     mark(4);
@@ -1580,14 +1626,16 @@ static hipError_t launch_impl(const BatchDesc& B, hipStream_t s, uint32_t raster
     else
         hipLaunchKernelGGL(k_raster<64>, dim3(raster_grid), dim3(64), 0, s, B);
     mark(6);
-    return hipGetLastError();
+    if (side) chk(hipStreamWaitEvent(s, side->join, 0));
+    chk(hipGetLastError());
+    return e;
 }
-
-hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid) {
-    return launch_impl(B, s, raster_grid, nullptr);
+hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, const SideStream* side) {
+    return launch_impl(B, s, raster_grid, nullptr, side);
 }
-hipError_t launch_pipeline_timed(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev) {
-    return launch_impl(B, s, raster_grid, ev);
+hipError_t launch_pipeline_timed(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev,
+                                 const SideStream* side) {
+    return launch_impl(B, s, raster_grid, ev, side);
 }
 
 }  // namespace rf

@@ -140,6 +140,7 @@ struct BatchDesc {
     uint32_t* tile_frame;  // per tile: its frame
     uint32_t* tile_list;   // tiles with runs, in tile order
     uint32_t* busy_count;  // their number
+    uint32_t clear_in_raster;  // 1: the rasterizer clears the tiles without runs itself
     Run* runs;
     uint32_t runs_cap;
     uint32_t* tile_counter;  // persistent raster work counters, one per tile partition (TILE_PARTS)
