@@ -907,10 +907,10 @@ constexpr int TILE_GRAB = RF_TILE_GRAB;  // tiles a wave takes from the shared c
 #endif
 constexpr int TILE_TAIL = RF_TILE_TAIL;  // the last tiles of a partition are taken one at a time
 #ifndef RF_KMAX
-#define RF_KMAX 160
+#define RF_KMAX 300
 #endif
 #ifndef RF_JSER
-#define RF_JSER 64
+#define RF_JSER 128
 #endif
 constexpr int KMAX = RF_KMAX;  // row steps taken serially; more use the exact fast-forward
 constexpr int JSER = RF_JSER;  // pixel steps taken serially; more use the exact fast-forward
@@ -961,7 +961,14 @@ __device__ __forceinline__ uint32_t texel_pixel(const BatchDesc& B, uint32_t fmt
 template <int K>
 __device__ __forceinline__ void advance(float (&v)[K], const float (&d)[K], uint32_t n, uint32_t nser, int nlive) {
     if (n <= nser) {
-        for (uint32_t q = 0; q < n; ++q) {
+        uint32_t q = 0;
+        for (; q + 4 <= n; q += 4) {  // four steps per trip: fewer loop branches
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int i = 0; i < K; ++i) v[i] = v[i] + d[i];
+        }
+        for (; q < n; ++q) {
 #pragma unroll
             for (int i = 0; i < K; ++i) v[i] = v[i] + d[i];
         }
@@ -1274,11 +1281,16 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         const float ed[6] = {T.dlx, T.drx, T.dlz, T.dla[0], T.dla[1], T.dla[2]};
                         advance<6>(ev, ed, (uint32_t)(a - ty0), KMAX, 3 + na);
                         EdgeRec* out = &s_rec[rec_base + (uint32_t)(a - (int32_t)y0 - ra)];
-                        for (int32_t y = a; y < b; ++y, ++out) {
-                            *out = EdgeRec{ev[0], ev[1], ev[2], ev[3], ev[4], ev[5]};
+                        int32_t y = a;
+                        for (; y + 2 <= b; y += 2, out += 2) {  // two rows per trip
+                            out[0] = EdgeRec{ev[0], ev[1], ev[2], ev[3], ev[4], ev[5]};
+#pragma unroll
+                            for (int i = 0; i < 6; ++i) ev[i] = ev[i] + ed[i];
+                            out[1] = EdgeRec{ev[0], ev[1], ev[2], ev[3], ev[4], ev[5]};
 #pragma unroll
                             for (int i = 0; i < 6; ++i) ev[i] = ev[i] + ed[i];
                         }
+                        if (y < b) *out = EdgeRec{ev[0], ev[1], ev[2], ev[3], ev[4], ev[5]};
                     }
                 }
                 __syncthreads();
