@@ -897,7 +897,10 @@ struct EdgeRec {
 #endif
 constexpr int REC_CAP = RF_REC_CAP;  // edge records per batch
 constexpr int FQ_CAP = 512;          // fan queue (ring); one run adds at most 64 * 7 fans
-constexpr int CELL_W = 8;            // pixel columns per ordering cell
+#ifndef RF_CELL_W
+#define RF_CELL_W 8
+#endif
+constexpr int CELL_W = RF_CELL_W;    // pixel columns per ordering cell
 #ifndef RF_TILE_GRAB
 #define RF_TILE_GRAB 1
 #endif
@@ -907,10 +910,10 @@ constexpr int TILE_GRAB = RF_TILE_GRAB;  // tiles a wave takes from the shared c
 #endif
 constexpr int TILE_TAIL = RF_TILE_TAIL;  // the last tiles of a partition are taken one at a time
 #ifndef RF_KMAX
-#define RF_KMAX 300
+#define RF_KMAX 600
 #endif
 #ifndef RF_JSER
-#define RF_JSER 128
+#define RF_JSER 256
 #endif
 constexpr int KMAX = RF_KMAX;  // row steps taken serially; more use the exact fast-forward
 constexpr int JSER = RF_JSER;  // pixel steps taken serially; more use the exact fast-forward
@@ -1385,10 +1388,14 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         cl1 = (xs + j1 - 1 - (int32_t)x0) / CELL_W;
                         const int32_t je = jc + CHUNK < j1 ? jc + CHUNK : j1;
                         cw1 = (xs + je - 1 - (int32_t)x0) / CELL_W;
-                        for (int32_t c = cl0; c <= cl1; ++c) atomicMin(&s_cell[row * NCG + c], key);
+#pragma unroll
+                        for (int32_t c = 0; c < NCG; ++c)
+                            if (c >= cl0 && c <= cl1) atomicMin(&s_cell[row * NCG + c], key);
                     }
                     bool win = have;
-                    for (int32_t c = cl0; c <= cw1; ++c) win = win && s_cell[row * NCG + c] == key;
+#pragma unroll
+                    for (int32_t c = 0; c < NCG; ++c)
+                        if (c >= cl0 && c <= cw1) win = win && s_cell[row * NCG + c] == key;
                     if (win) {
                         const int32_t rem = j1 - jc;
                         const uint32_t li = (uint32_t)(xs + jc - (int32_t)x0) * LD + row;
@@ -1476,7 +1483,9 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                         if (jc >= j1) have = false;
                     }
                     // release the claims (every claimant resets what it touched)
-                    for (int32_t c = cl0; c <= cl1; ++c) s_cell[row * NCG + c] = 0xFFFFFFFFu;
+#pragma unroll
+                    for (int32_t c = 0; c < NCG; ++c)
+                        if (c >= cl0 && c <= cl1) s_cell[row * NCG + c] = 0xFFFFFFFFu;
                 }
                 __syncthreads();
                 fq_head += nuse;
