@@ -107,6 +107,8 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
     if ((e = scr.ensure(nt * sizeof(ScrTri), counter))) return e;
     if ((e = extra.ensure(std::max<uint64_t>(B.extra_cap, 1) * sizeof(ScrTri), counter))) return e;
     if ((e = rect.ensure(nt * 2 * sizeof(uint32_t), counter))) return e;
+    if ((e = group_draw.ensure((nt / 64 + 1) * sizeof(uint32_t), counter))) return e;
+    if ((e = vblock_draw.ensure((nv / 256 + 1) * sizeof(uint32_t), counter))) return e;
     const uint64_t ntl = std::max<uint64_t>(B.n_tiles, 1);
     if ((e = run_count.ensure(ntl * sizeof(uint32_t), counter))) return e;
     if ((e = run_off.ensure((ntl + 1) * sizeof(uint32_t), counter))) return e;
@@ -124,6 +126,8 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
     B.scr = (ScrTri*)scr.p;
     B.extra = (ScrTri*)extra.p;
     B.rect = (uint32_t*)rect.p;
+    B.group_draw = (uint32_t*)group_draw.p;
+    B.vblock_draw = (uint32_t*)vblock_draw.p;
     B.run_count = (uint32_t*)run_count.p;
     B.run_off = (uint32_t*)run_off.p;
     B.run_cursor = (uint32_t*)run_cursor.p;
@@ -141,7 +145,7 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
 
 void Scratch::release(uint64_t* counter) {
     for (DevBuf* b : {&cv, &ca, &sv, &hdr, &scr, &extra, &rect, &run_count, &run_off, &run_cursor,
-                      &scan_blk, &tile_frame, &tile_list, &runs, &misc, &stats})
+                      &scan_blk, &tile_frame, &tile_list, &group_draw, &vblock_draw, &runs, &misc, &stats})
         b->release(counter);
 }
 

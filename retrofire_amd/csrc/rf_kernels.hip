@@ -98,6 +98,18 @@ __device__ __forceinline__ uint32_t find_draw_tri(const DrawDesc* d, uint32_t n,
     }
     return lo;
 }
+// The same searches from a hint: the last draw with base <= a key at or
+// before this one (draw bases never decrease); usually one load.
+__device__ __forceinline__ uint32_t draw_of_tri(const DrawDesc* d, uint32_t n, uint32_t hint, uint64_t g) {
+    uint32_t k = hint;
+    while (k + 1 < n && d[k + 1].tri_base <= g) ++k;
+    return k;
+}
+__device__ __forceinline__ uint32_t draw_of_vert(const DrawDesc* d, uint32_t n, uint32_t hint, uint64_t s) {
+    uint32_t k = hint;
+    while (k + 1 < n && d[k + 1].vert_base <= s) ++k;
+    return k;
+}
 __device__ __forceinline__ uint32_t find_draw_vert(const DrawDesc* d, uint32_t n, uint64_t s) {
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
@@ -113,7 +125,7 @@ __device__ __forceinline__ uint32_t find_draw_vert(const DrawDesc* d, uint32_t n
 __global__ __launch_bounds__(256) void k_vertex(BatchDesc B) {
     const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= B.n_verts) return;
-    const uint32_t di = find_draw_vert(B.draws, B.n_draws, slot);
+    const uint32_t di = draw_of_vert(B.draws, B.n_draws, B.vblock_draw[slot / 256], slot);
     const DrawDesc* D = &B.draws[di];
     const uint64_t i = slot - D->vert_base;
     const float px = D->pos[3 * i + 0], py = D->pos[3 * i + 1], pz = D->pos[3 * i + 2];
@@ -316,7 +328,7 @@ __global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
     bool has_rect = false;
     uint32_t frame = 0xFFFFFFFFu;
     if (valid) {
-        const uint32_t di = find_draw_tri(B.draws, B.n_draws, g);
+        const uint32_t di = draw_of_tri(B.draws, B.n_draws, B.group_draw[group], g);
         const DrawDesc* D = &B.draws[di];
         const FrameDesc& F = B.frames[D->frame];
         frame = D->frame;
@@ -546,7 +558,7 @@ __global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
         r1 = B.rect[2 * g + 1];
         valid = r0 != 0xFFFFFFFFu;
         if (valid) {
-            const uint32_t di = find_draw_tri(B.draws, B.n_draws, g);
+            const uint32_t di = draw_of_tri(B.draws, B.n_draws, B.group_draw[group], g);
             const FrameDesc& F = B.frames[B.draws[di].frame];
             base = F.tile_base;
             ntx = F.ntx;
@@ -1590,6 +1602,9 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
 __global__ void k_reset(BatchDesc B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < B.n_tiles) B.run_count[i] = 0;
+    // the draw of each group's first triangle and of each block's first vertex
+    if (i < B.n_groups) B.group_draw[i] = find_draw_tri(B.draws, B.n_draws, (uint64_t)i * WAVE);
+    if ((uint64_t)i * 256 < B.n_verts) B.vblock_draw[i] = find_draw_vert(B.draws, B.n_draws, (uint64_t)i * 256);
     if (i < B.n_frames) {
         B.stats[i].prims_o = 0;
         B.stats[i].verts_o = 0;
@@ -1610,7 +1625,10 @@ static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raste
                               const SideStream* side) {
     BatchDesc B = B0;
     B.clear_in_raster = side ? 0u : 1u;
-    const uint32_t reset_n = B.n_tiles > B.n_frames ? B.n_tiles : B.n_frames;
+    uint32_t reset_n = B.n_tiles > B.n_frames ? B.n_tiles : B.n_frames;
+    reset_n = reset_n > B.n_groups ? reset_n : B.n_groups;
+    const uint32_t n_vblocks = (uint32_t)((B.n_verts + 255) / 256);
+    reset_n = reset_n > n_vblocks ? reset_n : n_vblocks;
     auto mark = [&](int i) {
         if (ev) (void)hipEventRecord(ev[i], s);
     };

@@ -141,6 +141,8 @@ struct BatchDesc {
     uint32_t* tile_list;   // tiles with runs, in tile order
     uint32_t* busy_count;  // their number
     uint32_t clear_in_raster;  // 1: the rasterizer clears the tiles without runs itself
+    uint32_t* group_draw;   // per 64-triangle group: the draw of its first triangle
+    uint32_t* vblock_draw;  // per 256 vertex slots: the draw of the first
     Run* runs;
     uint32_t runs_cap;
     uint32_t* tile_counter;  // persistent raster work counters, one per tile partition (TILE_PARTS)
