@@ -90,6 +90,26 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) 
     return x - v;
 }
 
+// Rust's saturating float-to-integer casts (`as i32`, `as u32`, `as u8`) in
+// one instruction each: V_CVT_I32_F32 and V_CVT_U32_F32 truncate toward
+// zero, saturate out-of-range values (infinities included) and convert NaN to
+// 0, which is exactly the `as` semantics; a C cast would leave those cases
+// undefined.
+__device__ __forceinline__ int32_t cvt_i32(float x) {
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ __forceinline__ uint32_t cvt_u32(float x) {
+    uint32_t r;
+    asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ __forceinline__ uint32_t cvt_u8(float x) {
+    const uint32_t r = cvt_u32(x);
+    return r < 255u ? r : 255u;
+}
+
 __device__ __forceinline__ uint32_t find_draw_tri(const DrawDesc* d, uint32_t n, uint64_t g) {
     uint32_t lo = 0, hi = n;  // last draw with tri_base <= g
     while (hi - lo > 1) {
@@ -1349,9 +1369,9 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                             // 32-bit saturating casts: any value the 64-bit `as usize`
                             // keeps apart from these is beyond the target and fails
                             // the bounds check the same way
-                            const uint32_t cnt_f = as_u32(rx1 - rx0);
-                            const uint32_t xs32 = as_u32(rx0);
-                            uint32_t xe = as_u32(rx1);
+                            const uint32_t cnt_f = cvt_u32(rx1 - rx0);
+                            const uint32_t xs32 = cvt_u32(rx0);
+                            uint32_t xe = cvt_u32(rx1);
                             if (xe < xs32) xe = xs32;
                             int32_t jj0 = 0, jj1 = 0;
                             if (xe > F.width || (uint32_t)y >= F.height) {
@@ -1452,8 +1472,8 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                                 tg = TAG_COLOR;
 #pragma unroll
                                 for (int p = 0; p < CHUNK; ++p)
-                                    pay[p] = pack_pixel(F.fmt, as_u8(256.0f * v0[p]), as_u8(256.0f * v1[p]),
-                                                        as_u8(256.0f * v2[p]), 0xFF);
+                                    pay[p] = pack_pixel(F.fmt, cvt_u8(256.0f * v0[p]), cvt_u8(256.0f * v1[p]),
+                                                        cvt_u8(256.0f * v2[p]), 0xFF);
                             } else {
                                 tg = itag;
                                 const float tw = (float)ip0, th = (float)ip1;
@@ -1467,13 +1487,13 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                                         su = su > mu ? mu : su;
                                         sv = sv < 0.0f ? 0.0f : sv;
                                         sv = sv > mv ? mv : sv;
-                                        pay[p] = as_u32(floorf(sv)) * ip2 + as_u32(floorf(su));
+                                        pay[p] = cvt_u32(floorf(sv)) * ip2 + cvt_u32(floorf(su));
                                     }
                                 } else {  // SamplerRepeatPot (tex.rs:240-266)
 #pragma unroll
                                     for (int p = 0; p < CHUNK; ++p) {
-                                        const uint32_t ui = (uint32_t)as_i32(floorf(tw * (v0[p] / zz[p]))) & (ip0 - 1);
-                                        const uint32_t vi = (uint32_t)as_i32(floorf(th * (v1[p] / zz[p]))) & (ip1 - 1);
+                                        const uint32_t ui = (uint32_t)cvt_i32(floorf(tw * (v0[p] / zz[p]))) & (ip0 - 1);
+                                        const uint32_t vi = (uint32_t)cvt_i32(floorf(th * (v1[p] / zz[p]))) & (ip1 - 1);
                                         pay[p] = vi * ip2 + ui;
                                     }
                                 }

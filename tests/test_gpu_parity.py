@@ -222,3 +222,24 @@ def test_many_groups_in_one_tile(n):
     o, g = run_both(1, tris, pos, cols, mvp, vp, w, h, ctx=rf.Context(clear=True, face_cull=None))
     assert_same(o, g)
     assert g[2][6] > 500
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_extreme_attributes(kind):
+    """Varyings far outside their usual range (huge, infinite, NaN, negative
+    zero): the samplers' and the Gouraud shader's float-to-integer casts
+    saturate and map NaN to 0 as Rust's `as` does."""
+    rng = np.random.default_rng(40 + kind)
+    w, h = 96, 64
+    pos, attr, tris, mvp, vp = random_soup(rng, 200, w, h, kind)
+    specials = np.array([1e20, -1e20, np.inf, -np.inf, np.nan, -0.0, 3e9, -3e9, 1e-30, 300.0, -300.0],
+                        np.float32)
+    pick = rng.integers(0, len(specials), size=attr.shape)
+    mask = rng.uniform(size=attr.shape) < 0.3
+    attr = np.where(mask, specials[pick], attr).astype(np.float32)
+    tex = None
+    if kind >= 2:
+        tex = rng.integers(0, 256, size=(16, 32, 4), dtype=np.uint8)
+    o, g = run_both(kind, tris, pos, attr, mvp, vp, w, h, tex=tex, ctx=rf.Context(clear=True, face_cull=None))
+    assert_same(o, g)
+    assert g[2][6] > 0
