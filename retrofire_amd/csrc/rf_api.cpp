@@ -154,31 +154,15 @@ uint32_t raster_grid(const Device& d, uint32_t n_tiles) {
     return std::max(1u, std::min(n_tiles, g));
 }
 
-hipError_t SideStream::create() {
-    hipError_t e;
-    if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking))) return e;
-    if ((e = hipEventCreateWithFlags(&fork, hipEventDisableTiming))) return e;
-    return hipEventCreateWithFlags(&join, hipEventDisableTiming);
-}
-
-void SideStream::destroy() {
-    if (s) (void)hipStreamSynchronize(s);
-    if (join) (void)hipEventDestroy(join);
-    if (fork) (void)hipEventDestroy(fork);
-    if (s) (void)hipStreamDestroy(s);
-    s = nullptr;
-    fork = join = nullptr;
-}
-
 // Runs the pipeline on `s` and waits; grows the clip-fan and run capacities
 // and runs again when they were exceeded. `restore` re-initialises anything
 // the aborted run may have written (the targets) before a rerun.
-rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const SideStream* side,
-                    const std::vector<FrameStats*>&, std::vector<FrameStats>& stats_out, uint32_t* err_out,
+rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const std::vector<FrameStats*>&,
+                    std::vector<FrameStats>& stats_out, uint32_t* err_out,
                     const std::function<hipError_t()>& restore) {
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (sc.prepare(B, &d.bytes) != hipSuccess) return RF_E_DEVICE;
-        if (launch_pipeline(B, s, raster_grid(d, B.n_tiles), side) != hipSuccess) return RF_E_DEVICE;
+        if (launch_pipeline(B, s, raster_grid(d, B.n_tiles)) != hipSuccess) return RF_E_DEVICE;
         uint32_t misc[3];
         uint32_t total_runs = 0;
         if (hipMemcpyAsync(misc, sc.misc.p, sizeof(misc), hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -253,12 +237,6 @@ rf_status rf_device_open(int ordinal, rf_device** out) {
         delete d;
         return RF_E_DEVICE;
     }
-    if (d->side.create() != hipSuccess) {
-        d->side.destroy();
-        (void)hipStreamDestroy(d->stream);
-        delete d;
-        return RF_E_DEVICE;
-    }
     *out = reinterpret_cast<rf_device*>(d);
     return RF_OK;
 }
@@ -272,7 +250,6 @@ rf_status rf_device_close(rf_device* dev) {
     for (DevBuf* b : {&d->up_pos, &d->up_attr, &d->up_tris, &d->up_tex, &d->up_color, &d->up_depth,
                       &d->up_desc})
         b->release(&d->bytes);
-    d->side.destroy();
     (void)hipStreamDestroy(d->stream);
     delete d;
     return RF_OK;
@@ -393,7 +370,7 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
 
     std::vector<FrameStats> fs;
     uint32_t err = 0;
-    st = run_sized(d, d.plain, B, s, &d.side, {}, fs, &err, upload_target);
+    st = run_sized(d, d.plain, B, s, {}, fs, &err, upload_target);
     if (st) return st;
     d.plain_extra_cap = std::max(d.plain_extra_cap, B.extra_cap);
     d.plain_runs_cap = std::max(d.plain_runs_cap, B.runs_cap);

@@ -122,7 +122,7 @@ static rf_status build_descs(Batch& b, uint32_t tile_w) {
 static rf_status size_batch(Batch& b) {
     std::vector<FrameStats> fs;
     uint32_t err = 0;
-    rf_status st = run_sized(*b.dev, b.sc, b.B, b.stream, nullptr, {}, fs, &err, nullptr);
+    rf_status st = run_sized(*b.dev, b.sc, b.B, b.stream, {}, fs, &err, nullptr);
     if (st) return st;
     if ((st = map_err(err))) return st;
     return RF_OK;
@@ -131,7 +131,7 @@ static rf_status size_batch(Batch& b) {
 static float time_eager(Batch& b, int reps) {
     if (hipEventRecord(b.ev0, b.stream)) return -1.0f;
     for (int i = 0; i < reps; ++i)
-        if (launch_pipeline(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles), nullptr)) return -1.0f;
+        if (launch_pipeline(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles))) return -1.0f;
     if (hipEventRecord(b.ev1, b.stream) || hipEventSynchronize(b.ev1)) return -1.0f;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, b.ev0, b.ev1)) return -1.0f;
@@ -148,9 +148,7 @@ static void destroy_graph(Batch& b) {
 static rf_status capture(Batch& b) {
     destroy_graph(b);
     if (hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal)) return RF_E_DEVICE;
-    // a captured graph runs its nodes in order, so the rasterizer clears the
-    // tiles without runs itself rather than in a side branch
-    const hipError_t le = launch_pipeline(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles), nullptr);
+    const hipError_t le = launch_pipeline(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles));
     const hipError_t ee = hipStreamEndCapture(b.stream, &b.graph);
     if (le || ee || !b.graph) return RF_E_DEVICE;
     if (hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0)) return RF_E_DEVICE;
@@ -323,7 +321,7 @@ rf_status rf_batch_profile(rf_batch* bh, int iters, float* ms) {
     double acc[6] = {0, 0, 0, 0, 0, 0};
     rf_status st = RF_OK;
     for (int it = 0; it < iters && !st; ++it) {
-        if (launch_pipeline_timed(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles), ev, nullptr) ||
+        if (launch_pipeline_timed(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles), ev) ||
             hipEventSynchronize(ev[6])) {
             st = RF_E_DEVICE;
             break;

@@ -37,20 +37,10 @@ struct Scratch {
     void release(uint64_t* counter);
 };
 
-// A second stream with fork/join events: the pipeline clears the tiles that
-// nothing draws into there, concurrently with the rasterizer.
-struct SideStream {
-    hipStream_t s = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    hipError_t create();
-    void destroy();
-};
-
 struct Device {
     int ordinal = 0;
     int num_cu = 256;
     hipStream_t stream = nullptr;
-    SideStream side;
     uint64_t bytes = 0;  // device memory held
     Scratch plain;
     DevBuf up_pos, up_attr, up_tris, up_tex, up_color, up_depth, up_desc;
@@ -83,18 +73,14 @@ rf_status check_tex(uint32_t kind, const void* tex, uint32_t w, uint32_t h, uint
 rf_status map_err(uint32_t e);
 void set_grid(FrameDesc& f, uint32_t tile_w, uint32_t& tile_base);
 uint32_t raster_grid(const Device& d, uint32_t n_tiles);
-rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const SideStream* side,
-                    const std::vector<FrameStats*>&,
+rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const std::vector<FrameStats*>&,
                     std::vector<FrameStats>& stats_out, uint32_t* err_out,
                     const std::function<hipError_t()>& restore);
 
-// Enqueues the whole pipeline for batch `B` on stream `s` (rf_kernels.hip);
-// with `side`, tiles without runs are cleared on the side stream (forked from
-// and joined back to `s`), else by the rasterizer itself.
-hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, const SideStream* side);
+// Enqueues the whole pipeline for batch `B` on stream `s` (rf_kernels.hip).
+hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid);
 // Same, with an event recorded after each stage (for per-kernel timing);
 // `ev` holds 8 events.
-hipError_t launch_pipeline_timed(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev,
-                                 const SideStream* side);
+hipError_t launch_pipeline_timed(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev);
 
 }  // namespace rf

@@ -566,63 +566,6 @@ __global__ __launch_bounds__(1024) void k_scan_add(BatchDesc B) {
     B.run_cursor[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
-    const uint32_t lane = lane_id();
-    const uint32_t group = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    if (group >= B.n_groups) return;
-    const uint64_t g = (uint64_t)group * WAVE + lane;
-    bool valid = false;
-    uint32_t r0 = 0, r1 = 0, base = 0, ntx = 0;
-    if (g < B.n_tris) {
-        r0 = B.rect[2 * g];
-        r1 = B.rect[2 * g + 1];
-        valid = r0 != 0xFFFFFFFFu;
-        if (valid) {
-            const uint32_t di = draw_of_tri(B.draws, B.n_draws, B.group_draw[group], g);
-            const FrameDesc& F = B.frames[B.draws[di].frame];
-            base = F.tile_base;
-            ntx = F.ntx;
-        }
-    }
-    const uint32_t cap = B.runs_cap;
-    // The tiles of the group in increasing order (as in for_each_tile); lane k
-    // collects the k-th (tile, mask), and every 64 of them take their run slots
-    // with one round of atomics.
-    const uint32_t tx0 = r0 & 0xFFFF, tx1 = r1 & 0xFFFF, ty1 = r1 >> 16;
-    uint32_t cx = tx0, cy = r0 >> 16;
-    uint32_t my_tile = 0, k = 0;
-    unsigned long long my_mask = 0;
-    auto flush = [&]() {
-        if (lane < k) {
-            const uint32_t pos = B.run_off[my_tile] + atomicAdd(&B.run_cursor[my_tile], 1u);
-            if (pos < cap) {
-                Run r;
-                r.group = group;
-                r.pad = 0;
-                r.mask = my_mask;
-                B.runs[pos] = r;
-            }
-        }
-        k = 0;
-    };
-    while (true) {
-        const uint32_t next = valid ? base + cy * ntx + cx : 0xFFFFFFFFu;
-        const uint32_t t = wave_min_u32(next);
-        if (t == 0xFFFFFFFFu) break;
-        const bool mine = valid && next == t;
-        const unsigned long long m = __ballot(mine);
-        if (lane == k) {
-            my_tile = t;
-            my_mask = m;
-        }
-        if (++k == (uint32_t)WAVE) flush();
-        if (mine && ++cx > tx1) {
-            cx = tx0;
-            if (++cy > ty1) valid = false;
-        }
-    }
-    flush();
-}
 constexpr int SORT_LDS = 1024;  // bins up to this many runs sort in LDS
 
 // Sorts the runs held one per lane (lanes >= n hold none) by group: a bitonic
@@ -908,6 +851,139 @@ __device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint
 }
 
 
+#ifndef RF_CLEAR_IN_RASTER
+#define RF_CLEAR_IN_RASTER 1
+#endif
+#ifndef RF_EMPTY_GRAB
+#define RF_EMPTY_GRAB 16
+#endif
+constexpr int EMPTY_GRAB = RF_EMPTY_GRAB;  // tiles without runs a wave takes at once
+
+// The fused clear (Frame::clear) of one tile that nothing draws into: 16-byte
+// stores where whole rows of 4-byte pixels are aligned.
+template <int TW>
+__device__ __forceinline__ void clear_tile(const FrameDesc& F, uint32_t tile, uint32_t lane) {
+    const bool has_depth = F.depth != nullptr;
+    if (!F.clear_color && !(has_depth && F.clear_depth)) return;
+    const uint32_t lt = tile - F.tile_base;
+    const uint32_t x0 = (lt % F.ntx) * TW, y0 = (lt / F.ntx) * TILE_H;
+    const uint32_t ncols = min((uint32_t)TW, F.width - x0), nrows = min((uint32_t)TILE_H, F.height - y0);
+    const bool vec = ncols == (uint32_t)TW && F.fmt != PIX_RGB8 && ((uintptr_t)F.color & 15u) == 0 &&
+                     (F.color_stride & 15u) == 0 &&
+                     (!has_depth || (((uintptr_t)F.depth & 15u) == 0 && (F.depth_stride & 3u) == 0));
+    if (vec) {
+        constexpr uint32_t QR = TW / 4;  // quads per row
+        const uint32_t cp = F.clear_pixel;
+        const float cz = F.clear_depth_value;
+        for (uint32_t q = lane; q < nrows * QR; q += WAVE) {
+            const uint32_t r = q / QR, c = (q - r * QR) * 4;
+            if (F.clear_color)
+                *(uint4*)(F.color + (uint64_t)(y0 + r) * F.color_stride + 4ull * (x0 + c)) = make_uint4(cp, cp, cp, cp);
+            if (has_depth && F.clear_depth)
+                *(float4*)(F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 + c) = make_float4(cz, cz, cz, cz);
+        }
+    } else {
+        for (uint32_t i = lane; i < ncols * nrows; i += WAVE) {
+            const uint32_t r = i / ncols, c = i - r * ncols;
+            if (F.clear_color) store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
+            if (has_depth && F.clear_depth)
+                F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = F.clear_depth_value;
+        }
+    }
+}
+
+// Clears tiles without runs (listed after the others in B.tile_list): wave
+// `w` takes EMPTY_GRAB list entries.
+__device__ __forceinline__ void clear_empty_tiles(const BatchDesc& B, uint32_t w, uint32_t lane) {
+    const uint32_t n_busy = *B.busy_count, n_empty = B.n_tiles - n_busy;
+    const uint32_t e0 = w * EMPTY_GRAB;
+    if (e0 >= n_empty) return;
+    const uint32_t ne = min((uint32_t)EMPTY_GRAB, n_empty - e0);
+    uint32_t my_tile = 0, my_fi = 0;
+    if (lane < ne) {
+        my_tile = B.tile_list[n_busy + e0 + lane];
+        my_fi = B.tile_frame[my_tile];
+    }
+    uint32_t cur_fi = 0xFFFFFFFFu;
+    FrameDesc F;
+    for (uint32_t j = 0; j < ne; ++j) {
+        const uint32_t tile = (uint32_t)__builtin_amdgcn_readlane((int)my_tile, (int)j);
+        const uint32_t fi = (uint32_t)__builtin_amdgcn_readlane((int)my_fi, (int)j);
+        if (fi != cur_fi) {
+            F = B.frames[fi];
+            cur_fi = fi;
+        }
+        if (B.tile_w == 32)
+            clear_tile<32>(F, tile, lane);
+        else
+            clear_tile<64>(F, tile, lane);
+    }
+}
+
+// Blocks past the groups' clear the tiles without runs: binning waits on
+// atomics and loads while the clears only store, so they share the GPU well.
+__global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
+    const uint32_t lane = lane_id();
+    const uint32_t bin_blocks = (B.n_groups + 3) / 4;
+    if (blockIdx.x >= bin_blocks) {
+        if (!B.clear_in_raster) clear_empty_tiles(B, (blockIdx.x - bin_blocks) * 4 + threadIdx.x / WAVE, lane);
+        return;
+    }
+    const uint32_t group = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (group >= B.n_groups) return;
+    const uint64_t g = (uint64_t)group * WAVE + lane;
+    bool valid = false;
+    uint32_t r0 = 0, r1 = 0, base = 0, ntx = 0;
+    if (g < B.n_tris) {
+        r0 = B.rect[2 * g];
+        r1 = B.rect[2 * g + 1];
+        valid = r0 != 0xFFFFFFFFu;
+        if (valid) {
+            const uint32_t di = draw_of_tri(B.draws, B.n_draws, B.group_draw[group], g);
+            const FrameDesc& F = B.frames[B.draws[di].frame];
+            base = F.tile_base;
+            ntx = F.ntx;
+        }
+    }
+    const uint32_t cap = B.runs_cap;
+    // The tiles of the group in increasing order (as in for_each_tile); lane k
+    // collects the k-th (tile, mask), and every 64 of them take their run slots
+    // with one round of atomics.
+    const uint32_t tx0 = r0 & 0xFFFF, tx1 = r1 & 0xFFFF, ty1 = r1 >> 16;
+    uint32_t cx = tx0, cy = r0 >> 16;
+    uint32_t my_tile = 0, k = 0;
+    unsigned long long my_mask = 0;
+    auto flush = [&]() {
+        if (lane < k) {
+            const uint32_t pos = B.run_off[my_tile] + atomicAdd(&B.run_cursor[my_tile], 1u);
+            if (pos < cap) {
+                Run r;
+                r.group = group;
+                r.pad = 0;
+                r.mask = my_mask;
+                B.runs[pos] = r;
+            }
+        }
+        k = 0;
+    };
+    while (true) {
+        const uint32_t next = valid ? base + cy * ntx + cx : 0xFFFFFFFFu;
+        const uint32_t t = wave_min_u32(next);
+        if (t == 0xFFFFFFFFu) break;
+        const bool mine = valid && next == t;
+        const unsigned long long m = __ballot(mine);
+        if (lane == k) {
+            my_tile = t;
+            my_mask = m;
+        }
+        if (++k == (uint32_t)WAVE) flush();
+        if (mine && ++cx > tx1) {
+            cx = tx0;
+            if (++cy > ty1) valid = false;
+        }
+    }
+    flush();
+}
 // Per-triangle data of a batch slot (64 B).
 struct SlotP {
     float dv[2][4];        // d(z, a0, a1, a2)/dx of the upper and lower trapezoid
@@ -1024,74 +1100,6 @@ __device__ __forceinline__ int32_t wave_incl_max_i32(int32_t v) {
 __device__ __forceinline__ bool depth_pass_u(int32_t dt, float nz, float cur) {
     return (dt == DT_NONE) | ((dt == DT_LESS) & (cur < nz)) | ((dt == DT_EQUAL) & (cur == nz)) |
            ((dt == DT_GREATER) & (cur > nz));
-}
-
-#ifndef RF_EMPTY_GRAB
-#define RF_EMPTY_GRAB 16
-#endif
-constexpr int EMPTY_GRAB = RF_EMPTY_GRAB;  // tiles without runs a wave takes at once
-
-// The fused clear (Frame::clear) of one tile that nothing draws into: 16-byte
-// stores where whole rows of 4-byte pixels are aligned.
-template <int TW>
-__device__ __forceinline__ void clear_tile(const FrameDesc& F, uint32_t tile, uint32_t lane) {
-    const bool has_depth = F.depth != nullptr;
-    if (!F.clear_color && !(has_depth && F.clear_depth)) return;
-    const uint32_t lt = tile - F.tile_base;
-    const uint32_t x0 = (lt % F.ntx) * TW, y0 = (lt / F.ntx) * TILE_H;
-    const uint32_t ncols = min((uint32_t)TW, F.width - x0), nrows = min((uint32_t)TILE_H, F.height - y0);
-    const bool vec = ncols == (uint32_t)TW && F.fmt != PIX_RGB8 && ((uintptr_t)F.color & 15u) == 0 &&
-                     (F.color_stride & 15u) == 0 &&
-                     (!has_depth || (((uintptr_t)F.depth & 15u) == 0 && (F.depth_stride & 3u) == 0));
-    if (vec) {
-        constexpr uint32_t QR = TW / 4;  // quads per row
-        const uint32_t cp = F.clear_pixel;
-        const float cz = F.clear_depth_value;
-        for (uint32_t q = lane; q < nrows * QR; q += WAVE) {
-            const uint32_t r = q / QR, c = (q - r * QR) * 4;
-            if (F.clear_color)
-                *(uint4*)(F.color + (uint64_t)(y0 + r) * F.color_stride + 4ull * (x0 + c)) = make_uint4(cp, cp, cp, cp);
-            if (has_depth && F.clear_depth)
-                *(float4*)(F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 + c) = make_float4(cz, cz, cz, cz);
-        }
-    } else {
-        for (uint32_t i = lane; i < ncols * nrows; i += WAVE) {
-            const uint32_t r = i / ncols, c = i - r * ncols;
-            if (F.clear_color) store_pixel(F, x0 + c, y0 + r, F.clear_pixel);
-            if (has_depth && F.clear_depth)
-                F.depth[(uint64_t)(y0 + r) * F.depth_stride + x0 + c] = F.clear_depth_value;
-        }
-    }
-}
-
-// Clears the tiles without runs (listed after the others in B.tile_list):
-// one wave per EMPTY_GRAB list entries.
-__global__ __launch_bounds__(256) void k_clear_empty(BatchDesc B) {
-    const uint32_t lane = lane_id();
-    const uint32_t w = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    const uint32_t n_busy = *B.busy_count, n_empty = B.n_tiles - n_busy;
-    const uint32_t e0 = w * EMPTY_GRAB;
-    if (e0 >= n_empty) return;
-    const uint32_t ne = min((uint32_t)EMPTY_GRAB, n_empty - e0);
-    uint32_t my_tile = 0, my_fi = 0;
-    if (lane < ne) {
-        my_tile = B.tile_list[n_busy + e0 + lane];
-        my_fi = B.tile_frame[my_tile];
-    }
-    uint32_t cur_fi = 0xFFFFFFFFu;
-    FrameDesc F;
-    for (uint32_t j = 0; j < ne; ++j) {
-        const uint32_t tile = (uint32_t)__builtin_amdgcn_readlane((int)my_tile, (int)j);
-        const uint32_t fi = (uint32_t)__builtin_amdgcn_readlane((int)my_fi, (int)j);
-        if (fi != cur_fi) {
-            F = B.frames[fi];
-            cur_fi = fi;
-        }
-        if (B.tile_w == 32)
-            clear_tile<32>(F, tile, lane);
-        else
-            clear_tile<64>(F, tile, lane);
-    }
 }
 
 template <int TW>
@@ -1641,10 +1649,11 @@ __global__ void k_reset(BatchDesc B) {
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev,
-                              const SideStream* side) {
+static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev) {
     BatchDesc B = B0;
-    B.clear_in_raster = side ? 0u : 1u;
+    // the rasterizer's waves clear the tiles without runs at their end (faster,
+    // measured, than k_bin's extra blocks doing it beside the binning)
+    B.clear_in_raster = RF_CLEAR_IN_RASTER;
     uint32_t reset_n = B.n_tiles > B.n_frames ? B.n_tiles : B.n_frames;
     reset_n = reset_n > B.n_groups ? reset_n : B.n_groups;
     const uint32_t n_vblocks = (uint32_t)((B.n_verts + 255) / 256);
@@ -1667,16 +1676,13 @@ static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raste
     hipLaunchKernelGGL(k_scan_local, dim3(nb ? nb : 1), dim3(1024), 0, s, B);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, B);
     hipLaunchKernelGGL(k_scan_add, dim3(nb ? nb : 1), dim3(1024), 0, s, B);
-    if (side) {
-        // the tiles without runs are known now: clear them beside the rest
-        chk(hipEventRecord(side->fork, s));
-        chk(hipStreamWaitEvent(side->s, side->fork, 0));
-        hipLaunchKernelGGL(k_clear_empty, dim3((B.n_tiles + 4 * EMPTY_GRAB - 1) / (4 * EMPTY_GRAB)), dim3(256), 0,
-                           side->s, B);
-        chk(hipEventRecord(side->join, side->s));
-    }
     mark(3);
-    if (B.n_groups) hipLaunchKernelGGL(k_bin, dim3((B.n_groups + 3) / 4), dim3(256), 0, s, B);
+    {
+        const uint32_t bin_blocks = (B.n_groups + 3) / 4;
+        const uint32_t clear_blocks = B.clear_in_raster ? 0u : (B.n_tiles + 4 * EMPTY_GRAB - 1) / (4 * EMPTY_GRAB);
+        if (bin_blocks + clear_blocks)
+            hipLaunchKernelGGL(k_bin, dim3(bin_blocks + clear_blocks), dim3(256), 0, s, B);
+    }
     mark(4);
     hipLaunchKernelGGL(k_binsort, dim3((B.n_tiles + WAVE - 1) / WAVE), dim3(WAVE), 0, s, B);
     mark(5);
@@ -1685,16 +1691,14 @@ static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raste
     else
         hipLaunchKernelGGL(k_raster<64>, dim3(raster_grid), dim3(64), 0, s, B);
     mark(6);
-    if (side) chk(hipStreamWaitEvent(s, side->join, 0));
     chk(hipGetLastError());
     return e;
 }
-hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, const SideStream* side) {
-    return launch_impl(B, s, raster_grid, nullptr, side);
+hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid) {
+    return launch_impl(B, s, raster_grid, nullptr);
 }
-hipError_t launch_pipeline_timed(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev,
-                                 const SideStream* side) {
-    return launch_impl(B, s, raster_grid, ev, side);
+hipError_t launch_pipeline_timed(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev) {
+    return launch_impl(B, s, raster_grid, ev);
 }
 
 }  // namespace rf

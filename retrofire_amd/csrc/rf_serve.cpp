@@ -47,7 +47,6 @@ using FbKey = std::tuple<uint32_t, uint32_t, uint32_t, int>;
 
 struct Lane {
     hipStream_t stream = nullptr;
-    SideStream side;
     Scratch sc;
     DevBuf desc;
     uint8_t* h_desc = nullptr;  // pinned staging for descriptors
@@ -307,7 +306,7 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     if (l.sc.prepare(B, &s.bytes)) return RF_E_DEVICE;
     l.B = B;
     l.start_s = now_s();
-    if (launch_pipeline(B, l.stream, raster_grid(d, B.n_tiles), &l.side)) return RF_E_DEVICE;
+    if (launch_pipeline(B, l.stream, raster_grid(d, B.n_tiles))) return RF_E_DEVICE;
     if (hipMemcpyAsync(l.h_stats, l.sc.stats.p, fds.size() * sizeof(FrameStats), hipMemcpyDeviceToHost,
                        l.stream) ||
         hipMemcpyAsync(l.h_misc, l.sc.misc.p, 12, hipMemcpyDeviceToHost, l.stream))
@@ -451,11 +450,10 @@ rf_status rf_serve_open(rf_device* dev, uint64_t budget_bytes, uint32_t max_batc
     s->max_batch = max_batch;
     for (uint32_t i = 0; i < n_streams; ++i) {
         Lane* l = new Lane();
-        if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) || l->side.create() ||
+        if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) ||
             hipEventCreateWithFlags(&l->ev, hipEventBlockingSync | hipEventDisableTiming) ||
             hipHostMalloc((void**)&l->h_misc, 64, 0)) {
             if (l->ev) (void)hipEventDestroy(l->ev);
-            l->side.destroy();
             if (l->stream) (void)hipStreamDestroy(l->stream);
             delete l;
             rf_serve_close(reinterpret_cast<rf_serve*>(s));
@@ -551,7 +549,6 @@ rf_status rf_serve_close(rf_serve* sh) {
         if (l->h_stats) (void)hipHostFree(l->h_stats);
         if (l->h_misc) (void)hipHostFree(l->h_misc);
         if (l->ev) (void)hipEventDestroy(l->ev);
-        l->side.destroy();
         if (l->stream) (void)hipStreamDestroy(l->stream);
         delete l;
     }
