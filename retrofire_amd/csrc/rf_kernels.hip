@@ -1103,7 +1103,7 @@ __device__ __forceinline__ bool depth_pass_u(int32_t dt, float nz, float cur) {
 }
 
 template <int TW>
-__global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
+__global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
     constexpr int NPX = TW * LD;
     constexpr int NCG = TW / CELL_W;
@@ -1202,28 +1202,45 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
             }
             wave_sort_runs(nruns, lrun_g, lrun_m);
         }
-        for (uint32_t ri = 0; ri <= nruns; ++ri) {
-            // 1. append the next run's triangles (their fans) to the queue
-            if (ri < nruns) {
-                Run run;
-                if (nruns <= (uint32_t)WAVE) {
-                    run.group = __shfl(lrun_g, ri, WAVE);
-                    run.mask = __shfl(lrun_m, ri, WAVE);
-                } else {
-                    run = B.runs[off + ri];
+        // Batches of up to 64 fans follow one another without a pause: once a
+        // batch has handed out its last record, the next one is set up while
+        // the items in flight finish (an item keeps in registers all it needs
+        // of its slot and record). Order keys run on across the batches.
+        uint32_t ri = 0;          // next run to append
+        uint32_t next = 0;        // next record of the current batch to hand out
+        uint32_t nrec = 0;        // records of the current batch
+        uint32_t key_base = 0;    // order key of the current batch's first record
+        int32_t carry = -1;       // slot of the last record handed out
+        bool have = false;
+        uint32_t key = 0, row = 0, ikind = 0, ip0 = 0, ip1 = 0, ip2 = 0, itag = 0;
+        float dz = 0, d0 = 0, d1 = 0, d2 = 0, z = 0, c0 = 0, c1 = 0, c2 = 0;
+        int32_t xs = 0, jc = 0, j1 = 0;
+        for (;;) {
+            if (next >= nrec) {
+                // append runs until a full batch of fans is queued (or none are left)
+                while (ri < nruns && fq_tail - fq_head < (uint32_t)WAVE) {
+                    Run run;
+                    if (nruns <= (uint32_t)WAVE) {
+                        run.group = __shfl(lrun_g, ri, WAVE);
+                        run.mask = __shfl(lrun_m, ri, WAVE);
+                    } else {
+                        run = B.runs[off + ri];
+                    }
+                    const uint64_t g = (uint64_t)run.group * WAVE + lane;
+                    const bool has = (run.mask >> lane) & 1ull;
+                    const uint32_t nf = has ? (B.hdr[g] & 7u) : 0u;
+                    uint32_t tot;
+                    const uint32_t pre = wave_excl_scan(nf, &tot);
+                    for (uint32_t j = 0; j < nf; ++j) s_fq[(fq_tail + pre + j) % FQ_CAP] = ((uint32_t)g << 3) | j;
+                    fq_tail += tot;
+                    ++ri;
+                    __syncthreads();
                 }
-                const uint64_t g = (uint64_t)run.group * WAVE + lane;
-                const bool has = (run.mask >> lane) & 1ull;
-                const uint32_t nf = has ? (B.hdr[g] & 7u) : 0u;
-                uint32_t tot;
-                const uint32_t pre = wave_excl_scan(nf, &tot);
-                for (uint32_t j = 0; j < nf; ++j) s_fq[(fq_tail + pre + j) % FQ_CAP] = ((uint32_t)g << 3) | j;
-                fq_tail += tot;
-                __syncthreads();
-            }
-            // 2. consume full batches, and at the end whatever is left
-            while (fq_tail - fq_head >= (ri < nruns ? (uint32_t)WAVE : 1u)) {
                 const uint32_t avail = fq_tail - fq_head;
+                if (avail == 0) {
+                    if (__ballot(have) == 0ull) break;  // the tile is done
+                    goto rounds;
+                }
                 const uint32_t nb = avail < (uint32_t)WAVE ? avail : (uint32_t)WAVE;
                 // ---- setup: lane = triangle
                 const bool act = lane < nb;
@@ -1265,15 +1282,20 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                 const unsigned long long nf_mask = ~__ballot(act && fits);
                 const uint32_t nuse = nf_mask == 0ull ? (uint32_t)WAVE : (uint32_t)(__ffsll((long long)nf_mask) - 1);
                 const bool use = lane < nuse;
-                const uint32_t nrec = __shfl(rec_base + rows_in, nuse - 1, WAVE);
+                const uint32_t nrec_new = __shfl(rec_base + rows_in, nuse - 1, WAVE);
                 // draw tags are relative to the tile's first draw (draws only grow
                 // along the triangle order); resolve pending texels before the
                 // range would overflow
                 const uint32_t dfirst = __shfl(di, 0, WAVE), dlast = __shfl(di, nuse - 1, WAVE);
+                const bool rebase = dbase_set && dlast - dbase >= (uint32_t)TAG_COLOR;
+                if (rebase && __ballot(have) != 0ull) {
+                    // items in flight still tag with the old base: let them finish first
+                    goto rounds;
+                }
                 if (!dbase_set) {
                     dbase = dfirst;
                     dbase_set = true;
-                } else if (dlast - dbase >= (uint32_t)TAG_COLOR) {
+                } else if (rebase) {
                     for (uint32_t i = lane; i < npx; i += WAVE) {
                         const uint32_t r = i / ncols, c = i - r * ncols;
                         const uint32_t li = c * LD + r;
@@ -1285,7 +1307,7 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                     }
                     dbase = dfirst;
                 }
-                for (uint32_t i = lane; i < nrec; i += WAVE) s_own[i] = -1;
+                for (uint32_t i = lane; i < nrec_new; i += WAVE) s_own[i] = -1;
                 __syncthreads();
                 if (use) {
                     SlotP& P = s_slot[lane];
@@ -1338,199 +1360,192 @@ __global__ __launch_bounds__(64) void k_raster(BatchDesc B) {
                 }
                 __syncthreads();
 
-                // ---- spans
-                uint32_t next = 0;   // next record to hand out (wave-uniform)
-                int32_t carry = -1;  // slot of the last record handed out
-                bool have = false;
-                uint32_t key = 0, row = 0, ikind = 0, ip0 = 0, ip1 = 0, ip2 = 0, itag = 0;
-                float dz = 0, d0 = 0, d1 = 0, d2 = 0, z = 0, c0 = 0, c1 = 0, c2 = 0;
-                int32_t xs = 0, jc = 0, j1 = 0;
-                for (;;) {
-                    const unsigned long long need = __ballot(!have);
-                    if (next < nrec && (uint32_t)__popcll(need) >= (uint32_t)REFILL_MIN) {
-                        uint32_t cnt;
-                        const uint32_t rank = wave_excl_scan(have ? 0u : 1u, &cnt);
-                        const uint32_t k = next + rank;
-                        const bool valid = !have && k < nrec;
-                        int32_t f = wave_incl_max_i32(valid ? (int32_t)s_own[k] : -1);
-                        f = f > carry ? f : carry;
-                        f = f < 0 ? 0 : f;  // every valid record has a slot at or before it
-                        carry = __shfl(f, WAVE - 1, WAVE);  // the scan is non-decreasing
-                        next = next + cnt < nrec ? next + cnt : nrec;
-                        if (valid) {
-                            const EdgeRec R = s_rec[k];
-                            const SlotP& P = s_slot[f];
-                            const int32_t r = (int32_t)(P.ra_rec & 0xFFFFu) + (int32_t)(k - (P.ra_rec >> 16));
-                            const int32_t y = (int32_t)y0 + r;
-                            const int h = y < P.ysplit ? 0 : 1;
-                            ikind = P.kind_tag & 0xFFFFu;
-                            dz = P.dv[h][0];
-                            d0 = P.dv[h][1];
-                            d1 = P.dv[h][2];
-                            d2 = P.dv[h][3];
-                            // the span of this row (ScanlineIter::next, raster.rs:84-113)
-                            const float rx0 = round_up_to_half(R.lx), rx1 = round_up_to_half(R.rx);
-                            // v0.lerp(&v0.step(&dv_dx), x0 - v0.x)
-                            const float tw_ = rx0 - R.lx;
-                            float vv[4] = {R.lz + ((R.lz + dz) - R.lz) * tw_, R.l0 + ((R.l0 + d0) - R.l0) * tw_,
-                                           R.l1 + ((R.l1 + d1) - R.l1) * tw_, R.l2 + ((R.l2 + d2) - R.l2) * tw_};
-                            // 32-bit saturating casts: any value the 64-bit `as usize`
-                            // keeps apart from these is beyond the target and fails
-                            // the bounds check the same way
-                            const uint32_t cnt_f = cvt_u32(rx1 - rx0);
-                            const uint32_t xs32 = cvt_u32(rx0);
-                            uint32_t xe = cvt_u32(rx1);
-                            if (xe < xs32) xe = xs32;
-                            int32_t jj0 = 0, jj1 = 0;
-                            if (xe > F.width || (uint32_t)y >= F.height) {
-                                errs |= ERRB_BOUNDS;
-                            } else {
-                                if (xe > xs32 && (xs32 < (P.rxb & 0xFFFFu) || xe > (P.rxb >> 16))) errs |= ERRB_MARGIN;
-                                // frags.i: the part of [xs, xe) inside this tile
-                                const uint32_t pa = xs32 > x0 ? xs32 : x0;
-                                const uint32_t pb = xe < x0 + TW ? xe : x0 + TW;
-                                if (pb > pa) fi_acc += pb - pa;
-                                const uint32_t nfrag = cnt_f < xe - xs32 ? cnt_f : xe - xs32;
-                                xs = (int32_t)xs32;
-                                jj0 = (int32_t)x0 - xs > 0 ? (int32_t)x0 - xs : 0;
-                                jj1 = (int32_t)x0 + TW - xs;
-                                if (jj1 > (int32_t)nfrag) jj1 = (int32_t)nfrag;
-                            }
-                            if (jj0 < jj1) {
-                                const float dd[4] = {dz, d0, d1, d2};
-                                // the fragment iterator stepped jj0 times (Scanline::fragments)
-                                advance<4>(vv, dd, (uint32_t)jj0, JSER, 1 + kind_attrs(ikind));
-                                z = vv[0];
-                                c0 = vv[1];
-                                c1 = vv[2];
-                                c2 = vv[3];
-                                have = true;
-                                key = k;
-                                row = (uint32_t)r;
-                                jc = jj0;
-                                j1 = jj1;
-                                ip0 = P.p0;
-                                ip1 = P.p1;
-                                ip2 = P.p2;
-                                itag = P.kind_tag >> 16;
-                            }
-                        }
-                    }
-                    const unsigned long long busy = __ballot(have);
-                    if (busy == 0ull) {
-                        if (next >= nrec) break;
-                        continue;
-                    }
-                    // claim every cell of the remaining pixels
-                    int32_t cl0 = 0, cl1 = -1, cw1 = -1;
-                    if (have) {
-                        cl0 = (xs + jc - (int32_t)x0) / CELL_W;
-                        cl1 = (xs + j1 - 1 - (int32_t)x0) / CELL_W;
-                        const int32_t je = jc + CHUNK < j1 ? jc + CHUNK : j1;
-                        cw1 = (xs + je - 1 - (int32_t)x0) / CELL_W;
-#pragma unroll
-                        for (int32_t c = 0; c < NCG; ++c)
-                            if (c >= cl0 && c <= cl1) atomicMin(&s_cell[row * NCG + c], key);
-                    }
-                    bool win = have;
-#pragma unroll
-                    for (int32_t c = 0; c < NCG; ++c)
-                        if (c >= cl0 && c <= cw1) win = win && s_cell[row * NCG + c] == key;
-                    if (win) {
-                        const int32_t rem = j1 - jc;
-                        const uint32_t li = (uint32_t)(xs + jc - (int32_t)x0) * LD + row;
-                        float zz[CHUNK], v0[CHUNK], v1[CHUNK], v2[CHUNK], cur[CHUNK];
-                        bool ps[CHUNK];
-#pragma unroll
-                        for (int p = 0; p < CHUNK; ++p) {
-                            zz[p] = z;
-                            v0[p] = c0;
-                            v1[p] = c1;
-                            v2[p] = c2;
-                            z = z + dz;
-                            c0 = c0 + d0;
-                            c1 = c1 + d1;
-                            c2 = c2 + d2;
-                        }
-                        // the pixels of one chunk are distinct: read all depths first
-                        if (has_depth) {
-#pragma unroll
-                            for (int p = 0; p < CHUNK; ++p) cur[p] = s_dep[p < rem ? li + p * LD : li];
-                        } else {
-#pragma unroll
-                            for (int p = 0; p < CHUNK; ++p) cur[p] = 0.0f;
-                        }
-                        uint32_t npass = 0;
-#pragma unroll
-                        for (int p = 0; p < CHUNK; ++p) {
-                            ps[p] = (p < rem) & depth_pass_u(dtest, zz[p], cur[p]);
-                            npass += ps[p] ? 1u : 0u;
-                        }
-                        if (F.color_write) {
-                            fo_acc += npass;
-                            // colour sources of the chunk's pixels (the shader kind is
-                            // decided once per chunk)
-                            uint32_t pay[CHUNK];
-                            uint32_t tg;
-                            if (ikind == KIND_FLAT) {
-                                tg = TAG_COLOR;
-#pragma unroll
-                                for (int p = 0; p < CHUNK; ++p) pay[p] = ip0;
-                            } else if (ikind == KIND_GOURAUD) {  // Color3f::to_color4 (color.rs:253-263)
-                                tg = TAG_COLOR;
-#pragma unroll
-                                for (int p = 0; p < CHUNK; ++p)
-                                    pay[p] = pack_pixel(F.fmt, cvt_u8(256.0f * v0[p]), cvt_u8(256.0f * v1[p]),
-                                                        cvt_u8(256.0f * v2[p]), 0xFF);
-                            } else {
-                                tg = itag;
-                                const float tw = (float)ip0, th = (float)ip1;
-                                if (ikind == KIND_TEX_CLAMP) {  // SamplerClamp (tex.rs:280-303)
-                                    const float mu = tw - 1.0f, mv = th - 1.0f;
-#pragma unroll
-                                    for (int p = 0; p < CHUNK; ++p) {
-                                        // Scanline::fragments z_div (raster.rs:61-68, vec.rs:666-676)
-                                        float su = (v0[p] / zz[p]) * tw, sv = (v1[p] / zz[p]) * th;
-                                        su = su < 0.0f ? 0.0f : su;
-                                        su = su > mu ? mu : su;
-                                        sv = sv < 0.0f ? 0.0f : sv;
-                                        sv = sv > mv ? mv : sv;
-                                        pay[p] = cvt_u32(floorf(sv)) * ip2 + cvt_u32(floorf(su));
-                                    }
-                                } else {  // SamplerRepeatPot (tex.rs:240-266)
-#pragma unroll
-                                    for (int p = 0; p < CHUNK; ++p) {
-                                        const uint32_t ui = (uint32_t)cvt_i32(floorf(tw * (v0[p] / zz[p]))) & (ip0 - 1);
-                                        const uint32_t vi = (uint32_t)cvt_i32(floorf(th * (v1[p] / zz[p]))) & (ip1 - 1);
-                                        pay[p] = vi * ip2 + ui;
-                                    }
-                                }
-                            }
-#pragma unroll
-                            for (int p = 0; p < CHUNK; ++p) {
-                                if (ps[p]) {
-                                    s_tag[li + p * LD] = (uint16_t)tg;
-                                    s_pay[li + p * LD] = pay[p];
-                                }
-                            }
-                        }
-                        if (dwrite) {
-#pragma unroll
-                            for (int p = 0; p < CHUNK; ++p)
-                                if (ps[p]) s_dep[li + p * LD] = zz[p];
-                        }
-                        jc += CHUNK;
-                        if (jc >= j1) have = false;
-                    }
-                    // release the claims (every claimant resets what it touched)
-#pragma unroll
-                    for (int32_t c = 0; c < NCG; ++c)
-                        if (c >= cl0 && c <= cl1) s_cell[row * NCG + c] = 0xFFFFFFFFu;
-                }
-                __syncthreads();
+                key_base += nrec;
+                nrec = nrec_new;
+                next = 0;
+                carry = -1;
                 fq_head += nuse;
             }
-        }
+        rounds:
+            const unsigned long long need = __ballot(!have);
+            if (next < nrec && (uint32_t)__popcll(need) >= (uint32_t)REFILL_MIN) {
+                uint32_t cnt;
+                const uint32_t rank = wave_excl_scan(have ? 0u : 1u, &cnt);
+                const uint32_t k = next + rank;
+                const bool valid = !have && k < nrec;
+                int32_t f = wave_incl_max_i32(valid ? (int32_t)s_own[k] : -1);
+                f = f > carry ? f : carry;
+                f = f < 0 ? 0 : f;  // every valid record has a slot at or before it
+                carry = __shfl(f, WAVE - 1, WAVE);  // the scan is non-decreasing
+                next = next + cnt < nrec ? next + cnt : nrec;
+                if (valid) {
+                    const EdgeRec R = s_rec[k];
+                    const SlotP& P = s_slot[f];
+                    const int32_t r = (int32_t)(P.ra_rec & 0xFFFFu) + (int32_t)(k - (P.ra_rec >> 16));
+                    const int32_t y = (int32_t)y0 + r;
+                    const int h = y < P.ysplit ? 0 : 1;
+                    ikind = P.kind_tag & 0xFFFFu;
+                    dz = P.dv[h][0];
+                    d0 = P.dv[h][1];
+                    d1 = P.dv[h][2];
+                    d2 = P.dv[h][3];
+                    // the span of this row (ScanlineIter::next, raster.rs:84-113)
+                    const float rx0 = round_up_to_half(R.lx), rx1 = round_up_to_half(R.rx);
+                    // v0.lerp(&v0.step(&dv_dx), x0 - v0.x)
+                    const float tw_ = rx0 - R.lx;
+                    float vv[4] = {R.lz + ((R.lz + dz) - R.lz) * tw_, R.l0 + ((R.l0 + d0) - R.l0) * tw_,
+                                   R.l1 + ((R.l1 + d1) - R.l1) * tw_, R.l2 + ((R.l2 + d2) - R.l2) * tw_};
+                    // 32-bit saturating casts: any value the 64-bit `as usize`
+                    // keeps apart from these is beyond the target and fails
+                    // the bounds check the same way
+                    const uint32_t cnt_f = cvt_u32(rx1 - rx0);
+                    const uint32_t xs32 = cvt_u32(rx0);
+                    uint32_t xe = cvt_u32(rx1);
+                    if (xe < xs32) xe = xs32;
+                    int32_t jj0 = 0, jj1 = 0;
+                    if (xe > F.width || (uint32_t)y >= F.height) {
+                        errs |= ERRB_BOUNDS;
+                    } else {
+                        if (xe > xs32 && (xs32 < (P.rxb & 0xFFFFu) || xe > (P.rxb >> 16))) errs |= ERRB_MARGIN;
+                        // frags.i: the part of [xs, xe) inside this tile
+                        const uint32_t pa = xs32 > x0 ? xs32 : x0;
+                        const uint32_t pb = xe < x0 + TW ? xe : x0 + TW;
+                        if (pb > pa) fi_acc += pb - pa;
+                        const uint32_t nfrag = cnt_f < xe - xs32 ? cnt_f : xe - xs32;
+                        xs = (int32_t)xs32;
+                        jj0 = (int32_t)x0 - xs > 0 ? (int32_t)x0 - xs : 0;
+                        jj1 = (int32_t)x0 + TW - xs;
+                        if (jj1 > (int32_t)nfrag) jj1 = (int32_t)nfrag;
+                    }
+                    if (jj0 < jj1) {
+                        const float dd[4] = {dz, d0, d1, d2};
+                        // the fragment iterator stepped jj0 times (Scanline::fragments)
+                        advance<4>(vv, dd, (uint32_t)jj0, JSER, 1 + kind_attrs(ikind));
+                        z = vv[0];
+                        c0 = vv[1];
+                        c1 = vv[2];
+                        c2 = vv[3];
+                        have = true;
+                        key = key_base + k;
+                        row = (uint32_t)r;
+                        jc = jj0;
+                        j1 = jj1;
+                        ip0 = P.p0;
+                        ip1 = P.p1;
+                        ip2 = P.p2;
+                        itag = P.kind_tag >> 16;
+                    }
+                }
+            }
+            const unsigned long long busy = __ballot(have);
+            if (busy == 0ull) continue;
+            // claim every cell of the remaining pixels
+            int32_t cl0 = 0, cl1 = -1, cw1 = -1;
+            if (have) {
+                cl0 = (xs + jc - (int32_t)x0) / CELL_W;
+                cl1 = (xs + j1 - 1 - (int32_t)x0) / CELL_W;
+                const int32_t je = jc + CHUNK < j1 ? jc + CHUNK : j1;
+                cw1 = (xs + je - 1 - (int32_t)x0) / CELL_W;
+#pragma unroll
+                for (int32_t c = 0; c < NCG; ++c)
+                    if (c >= cl0 && c <= cl1) atomicMin(&s_cell[row * NCG + c], key);
+            }
+            bool win = have;
+#pragma unroll
+            for (int32_t c = 0; c < NCG; ++c)
+                if (c >= cl0 && c <= cw1) win = win && s_cell[row * NCG + c] == key;
+            if (win) {
+                const int32_t rem = j1 - jc;
+                const uint32_t li = (uint32_t)(xs + jc - (int32_t)x0) * LD + row;
+                float zz[CHUNK], v0[CHUNK], v1[CHUNK], v2[CHUNK], cur[CHUNK];
+                bool ps[CHUNK];
+#pragma unroll
+                for (int p = 0; p < CHUNK; ++p) {
+                    zz[p] = z;
+                    v0[p] = c0;
+                    v1[p] = c1;
+                    v2[p] = c2;
+                    z = z + dz;
+                    c0 = c0 + d0;
+                    c1 = c1 + d1;
+                    c2 = c2 + d2;
+                }
+                // the pixels of one chunk are distinct: read all depths first
+                if (has_depth) {
+#pragma unroll
+                    for (int p = 0; p < CHUNK; ++p) cur[p] = s_dep[p < rem ? li + p * LD : li];
+                } else {
+#pragma unroll
+                    for (int p = 0; p < CHUNK; ++p) cur[p] = 0.0f;
+                }
+                uint32_t npass = 0;
+#pragma unroll
+                for (int p = 0; p < CHUNK; ++p) {
+                    ps[p] = (p < rem) & depth_pass_u(dtest, zz[p], cur[p]);
+                    npass += ps[p] ? 1u : 0u;
+                }
+                if (F.color_write) {
+                    fo_acc += npass;
+                    // colour sources of the chunk's pixels (the shader kind is
+                    // decided once per chunk)
+                    uint32_t pay[CHUNK];
+                    uint32_t tg;
+                    if (ikind == KIND_FLAT) {
+                        tg = TAG_COLOR;
+#pragma unroll
+                        for (int p = 0; p < CHUNK; ++p) pay[p] = ip0;
+                    } else if (ikind == KIND_GOURAUD) {  // Color3f::to_color4 (color.rs:253-263)
+                        tg = TAG_COLOR;
+#pragma unroll
+                        for (int p = 0; p < CHUNK; ++p)
+                            pay[p] = pack_pixel(F.fmt, cvt_u8(256.0f * v0[p]), cvt_u8(256.0f * v1[p]),
+                                                cvt_u8(256.0f * v2[p]), 0xFF);
+                    } else {
+                        tg = itag;
+                        const float tw = (float)ip0, th = (float)ip1;
+                        if (ikind == KIND_TEX_CLAMP) {  // SamplerClamp (tex.rs:280-303)
+                            const float mu = tw - 1.0f, mv = th - 1.0f;
+#pragma unroll
+                            for (int p = 0; p < CHUNK; ++p) {
+                                // Scanline::fragments z_div (raster.rs:61-68, vec.rs:666-676)
+                                float su = (v0[p] / zz[p]) * tw, sv = (v1[p] / zz[p]) * th;
+                                su = su < 0.0f ? 0.0f : su;
+                                su = su > mu ? mu : su;
+                                sv = sv < 0.0f ? 0.0f : sv;
+                                sv = sv > mv ? mv : sv;
+                                pay[p] = cvt_u32(floorf(sv)) * ip2 + cvt_u32(floorf(su));
+                            }
+                        } else {  // SamplerRepeatPot (tex.rs:240-266)
+#pragma unroll
+                            for (int p = 0; p < CHUNK; ++p) {
+                                const uint32_t ui = (uint32_t)cvt_i32(floorf(tw * (v0[p] / zz[p]))) & (ip0 - 1);
+                                const uint32_t vi = (uint32_t)cvt_i32(floorf(th * (v1[p] / zz[p]))) & (ip1 - 1);
+                                pay[p] = vi * ip2 + ui;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int p = 0; p < CHUNK; ++p) {
+                        if (ps[p]) {
+                            s_tag[li + p * LD] = (uint16_t)tg;
+                            s_pay[li + p * LD] = pay[p];
+                        }
+                    }
+                }
+                if (dwrite) {
+#pragma unroll
+                    for (int p = 0; p < CHUNK; ++p)
+                        if (ps[p]) s_dep[li + p * LD] = zz[p];
+                }
+                jc += CHUNK;
+                if (jc >= j1) have = false;
+            }
+            // release the claims (every claimant resets what it touched)
+#pragma unroll
+            for (int32_t c = 0; c < NCG; ++c)
+                if (c >= cl0 && c <= cl1) s_cell[row * NCG + c] = 0xFFFFFFFFu;
+                }
+        __syncthreads();
 
         // write-back: fetch the texels, colour and depth to the target
         if (vec && (F.clear_color || F.color_write)) {
