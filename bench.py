@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--tile-w", type=int, default=0, help="0 = autotune")
     p.add_argument("--serve-seconds", type=float, default=2.0)
     p.add_argument("--serve-rate", type=float, default=1000.0)
+    p.add_argument("--serve-sweep-seconds", type=float, default=1.0,
+                   help="seconds per offered rate of the 16-mesh serving sweep (0 = no sweep)")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-serve", action="store_true")
@@ -202,35 +204,38 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def serve_run(rf, RB, mesh, texr, shd, a):
-    from retrofire_amd import mat as M
-    from retrofire_amd import scenes as S
-    rng = S.Xorshift64(S.DEFAULT_SEED + 4)
-    res_list = [(256, 256), (512, 512), (1280, 720), (1920, 1080), (2048, 2048)]
+RES_LIST = [(256, 256), (512, 512), (1280, 720), (1920, 1080), (2048, 2048)]
+
+
+def crates_request(S, M, RB, mesh, shd, texr, w, h, yaw):
+    aspect = np.float32(w) / np.float32(h)
+    proj = M.perspective(M.fov_diagonal_focal_ratio(90.0, aspect), aspect, 0.1, 1000.0)
+    mvp = M.then(S.first_person(0.0, 1.7, 0.0, yaw), proj)
+    return [RB.mesh_draw(mesh, shd, mvp, M.viewport(0, h, w, 0), texture=texr)]
+
+
+def spheres_request(S, M, RB, mesh, shd, w, h):
+    mvp = M.then(M.translate3(0.0, 0.0, 2.5), M.perspective(np.float32(1.7320508), np.float32(w) / np.float32(h),
+                                                            0.1, 100.0))
+    return [RB.mesh_draw(mesh, shd, mvp, M.viewport(0, h, w, 0))]
+
+
+def serve_stream(RB, reqs, seconds, warm):
+    """Open-loop replay of (time, w, h, draws) requests into a fresh server;
+    latency = completion - submission. `warm` (w, h, draws) requests run
+    first, untimed, so buffer pools and scratch reach their steady size."""
     srv = RB.Server(budget_bytes=4 << 30, max_batch=32, n_streams=3)
-    n = max(1, int(a.serve_rate * a.serve_seconds))
-    reqs = []
-    t = 0.0
-    for i in range(n):
-        u = float(rng.uniform_f32(0.0, 1.0))
-        t += -math.log(max(1e-7, 1.0 - u)) / a.serve_rate
-        w, h = res_list[rng.uniform_usize(0, 5)]
-        yaw = float(rng.uniform_f32(0.0, 1.0))
-        aspect = np.float32(w) / np.float32(h)
-        proj = M.perspective(M.fov_diagonal_focal_ratio(90.0, aspect), aspect, 0.1, 1000.0)
-        mvp = M.then(S.first_person(0.0, 1.7, 0.0, yaw), proj)
-        reqs.append((t, w, h, [RB.mesh_draw(mesh, shd, mvp, M.viewport(0, h, w, 0), texture=texr)]))
-    # warm the pools with one of each resolution
-    for k, (w, h) in enumerate(res_list):
-        srv.submit(10 ** 9 + k, reqs[0][3], w, h)
+    n = len(reqs)
+    for k, (w, h, draws) in enumerate(warm):
+        srv.submit(10 ** 9 + k, draws, w, h)
     got = 0
     t_warm = time.perf_counter() + 60.0
-    while got < len(res_list) and time.perf_counter() < t_warm:
+    while got < len(warm) and time.perf_counter() < t_warm:
         got += len(srv.poll(64, timeout_ms=1000))
     t0 = time.perf_counter()
     done = []
     i = 0
-    deadline = t0 + a.serve_seconds + 60.0
+    deadline = t0 + seconds + 60.0
     while len(done) < n and time.perf_counter() < deadline:
         now = time.perf_counter() - t0
         while i < n and reqs[i][0] <= now:
@@ -244,13 +249,64 @@ def serve_run(rf, RB, mesh, texr, shd, a):
                 time.sleep(min(nxt, 0.001))
     t1 = time.perf_counter()
     srv.close()
-    lat = np.array([(r.done_s - r.submit_s) for r in done]) * 1e3
-    ok = sum(1 for r in done if r.status == 0)
     if len(done) < n:
         return {"requests": n, "completed": len(done), "error": "serving run did not finish"}
-    return {"requests": n, "ok": ok, "offered_rate": a.serve_rate, "frames_per_s": round(n / (t1 - t0), 1),
-            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "resolutions": "uniform over 256^2, 512^2, 1280x720, 1920x1080, 2048^2"}
+    lat = np.array([(r.done_s - r.submit_s) for r in done]) * 1e3
+    ok = sum(1 for r in done if r.status == 0)
+    return {"requests": n, "ok": ok, "frames_per_s": round(n / (t1 - t0), 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+
+
+def serve_run(rf, RB, mesh, texr, shd, a):
+    """Serving: open-loop Poisson arrivals. Headline at --serve-rate over the
+    crates1080 mesh at the five resolutions; then a sweep of offered rates
+    over 16 resident meshes (8 crate fields of 125..1000 crates, 8 icosphere
+    clouds of 250..2000 spheres), each request one mesh at one resolution."""
+    from retrofire_amd import mat as M
+    from retrofire_amd import scenes as S
+    rng = S.Xorshift64(S.DEFAULT_SEED + 4)
+    n = max(1, int(a.serve_rate * a.serve_seconds))
+    reqs, t = [], 0.0
+    for i in range(n):
+        t += -math.log(max(1e-7, 1.0 - float(rng.uniform_f32(0.0, 1.0)))) / a.serve_rate
+        w, h = RES_LIST[rng.uniform_usize(0, 5)]
+        reqs.append((t, w, h, crates_request(S, M, RB, mesh, shd, texr, w, h, float(rng.uniform_f32(0.0, 1.0)))))
+    out = serve_stream(RB, reqs, a.serve_seconds, [(w, h, reqs[0][3]) for (w, h) in RES_LIST])
+    out["offered_rate"] = a.serve_rate
+    out["resolutions"] = "uniform over 256^2, 512^2, 1280x720, 1920x1080, 2048^2"
+    if a.serve_sweep_seconds <= 0:
+        return out
+    # 16 resident meshes
+    meshes = []
+    for k in range(8):
+        pos, uv, tris, _ = S.crates_world(125 * (k + 1), seed=S.DEFAULT_SEED + 100 + k)
+        meshes.append(("crates", RB.Mesh(pos, tris, uv)))
+    gour = rf.shader.Gouraud()
+    for k in range(8):
+        pos, attr, tris, _, _ = S.icospheres_world(250 * (k + 1), seed=S.DEFAULT_SEED + 200 + k)
+        meshes.append(("spheres", RB.Mesh(pos, tris, attr)))
+    sweep = []
+    for rate in (200.0, 500.0, 1000.0, 2000.0):
+        rng = S.Xorshift64(S.DEFAULT_SEED + 5)
+        m = max(1, int(rate * a.serve_sweep_seconds))
+        reqs, t = [], 0.0
+        for i in range(m):
+            t += -math.log(max(1e-7, 1.0 - float(rng.uniform_f32(0.0, 1.0)))) / rate
+            w, h = RES_LIST[rng.uniform_usize(0, 5)]
+            kind, me = meshes[rng.uniform_usize(0, 16)]
+            yaw = float(rng.uniform_f32(0.0, 1.0))
+            draws = (crates_request(S, M, RB, me, shd, texr, w, h, yaw) if kind == "crates"
+                     else spheres_request(S, M, RB, me, gour, w, h))
+            reqs.append((t, w, h, draws))
+        warm = [(w, h, crates_request(S, M, RB, meshes[7][1], shd, texr, w, h, 0.0)) for (w, h) in RES_LIST]
+        warm += [(w, h, spheres_request(S, M, RB, meshes[15][1], gour, w, h)) for (w, h) in RES_LIST]
+        r = serve_stream(RB, reqs, a.serve_sweep_seconds, warm)
+        r["offered_rate"] = rate
+        sweep.append(r)
+    for _, me in meshes:
+        me.free()
+    out["sweep_16_meshes"] = sweep
+    return out
 
 
 def cpu_baseline(pos, uv, tris, tex, mvps, vp, W, H, seconds):

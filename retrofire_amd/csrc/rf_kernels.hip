@@ -1173,13 +1173,17 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
             continue;
         }
 
-        // tile init (lane = row): depth from the fused clear or the target; no
-        // fragment yet
-        if (lane < nrows) {
-            const float* drow = has_depth ? F.depth + (uint64_t)(y0 + lane) * F.depth_stride + x0 : nullptr;
-            for (uint32_t c = 0; c < ncols; ++c) {
-                if (has_depth) s_dep[c * LD + lane] = F.clear_depth ? F.clear_depth_value : drow[c];
-                s_tag[c * LD + lane] = TAG_NONE;
+        // tile init (WAVE / TILE_H lanes per row, each a share of the columns):
+        // depth from the fused clear or the target; no fragment yet
+        {
+            constexpr uint32_t LPR = WAVE / TILE_H;  // lanes per row
+            const uint32_t r = lane % TILE_H, part = lane / TILE_H;
+            if (r < nrows) {
+                const float* drow = has_depth ? F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 : nullptr;
+                for (uint32_t c = part; c < ncols; c += LPR) {
+                    if (has_depth) s_dep[c * LD + r] = F.clear_depth ? F.clear_depth_value : drow[c];
+                    s_tag[c * LD + r] = TAG_NONE;
+                }
             }
         }
         __syncthreads();

@@ -61,6 +61,12 @@ def test_crates1080_parity():
     assert g[2][6] > 100000
 
 
+def test_icospheres_full_parity():
+    """The 10,000-icosphere scene at full size (3.2M triangles, 1920x1080)."""
+    o, g = _scene_parity(S.icospheres1080())
+    assert g[2][1] == 3200000 and g[2][6] > 1000000
+
+
 def test_icospheres_parity_reduced():
     sc = S.icospheres1080(n=1500)
     o, g = _scene_parity(sc)
