@@ -55,12 +55,19 @@ typedef enum rf_shader_kind {
     RF_SHADER_TEX_REPEAT_POT = 3   /* TexCoord varying, SamplerRepeatPot (tex.rs:218)    */
 } rf_shader_kind;
 
-/* Colour buffer element layouts (target.rs:35-136, pixfmt.rs:76-97). */
+/* Colour buffer element layouts (target.rs:35-136, pixfmt.rs:58-138). The
+ * u32 forms are stored little-endian, so RF_PIX_U32_ARGB is also the byte
+ * layout of Colorbuf<Buf2<[u8; 4]>, Bgra8888> and RF_PIX_U32_BGRA that of
+ * Colorbuf<Buf2<[u8; 4]>, Argb8888>. The u16 forms take 2 bytes a pixel. */
 typedef enum rf_pixfmt {
     RF_PIX_RGBA8 = 0,      /* [u8; 4] r,g,b,a: Buf2<Color4> or Colorbuf<_, Rgba8888> */
     RF_PIX_RGB8 = 1,       /* [u8; 3] r,g,b:   Buf2<Color3>                          */
     RF_PIX_U32_RGBA = 2,   /* u32 0xRRGGBBAA:  Colorbuf<Buf2<u32>, Rgba8888>          */
-    RF_PIX_U32_XRGB = 3    /* u32 0x00RRGGBB:  Colorbuf<Buf2<u32>, Xrgb8888>          */
+    RF_PIX_U32_XRGB = 3,   /* u32 0x00RRGGBB:  Colorbuf<Buf2<u32>, Xrgb8888>          */
+    RF_PIX_U32_ARGB = 4,   /* u32 0xAARRGGBB:  Colorbuf<Buf2<u32>, Argb8888>          */
+    RF_PIX_U32_BGRA = 5,   /* u32 0xBBGGRRAA:  Colorbuf<Buf2<u32>, Bgra8888>          */
+    RF_PIX_RGBA4444 = 6,   /* u16 rrrrggggbbbbaaaa: Colorbuf<Buf2<u16>, Rgba4444>     */
+    RF_PIX_RGB565 = 7      /* u16 rrrrrggggggbbbbb: Colorbuf<Buf2<u16>, Rgb565>       */
 } rf_pixfmt;
 
 /* Context (ctx.rs:11-65). depth_test holds the reference's

@@ -455,7 +455,13 @@ enum PixFmt {
     PF_RGB8 = 1,       // [u8;3] r,g,b: Buf2<Color3> via Color4::to_rgb (target.rs:127-136)
     PF_U32_RGBA = 2,   // u32 0xRRGGBBAA: Rgba8888 into u32 (pixfmt.rs:76-84)
     PF_U32_XRGB = 3,   // u32 0x00RRGGBB: Xrgb8888 (pixfmt.rs:86-92)
+    PF_U32_ARGB = 4,   // u32 0xAARRGGBB: Argb8888 into u32; in memory the Bgra8888 bytes (pixfmt.rs:99-104)
+    PF_U32_BGRA = 5,   // u32 0xBBGGRRAA: Bgra8888 into u32; in memory the Argb8888 bytes (pixfmt.rs:105-110)
+    PF_RGBA4444 = 6,   // u16 r,g,b,a nibbles from the top: Rgba4444 (pixfmt.rs:122-133)
+    PF_RGB565 = 7,     // u16 5,6,5 bits r,g,b: Rgb565 of Color4::to_rgb (pixfmt.rs:58-65, 134-138)
 };
+
+inline uint64_t pixfmt_bytes(int fmt) { return fmt == PF_RGB8 ? 3 : (fmt >= PF_RGBA4444 ? 2 : 4); }
 
 struct Target {
     uint8_t* color = nullptr;   // row-major, `color_stride` bytes per row
@@ -490,6 +496,26 @@ inline void store_pixel(const Target& t, uint64_t x, uint64_t y, Rgba c) {
         case PF_U32_XRGB: {
             uint32_t v = ((uint32_t)c.r << 16) | ((uint32_t)c.g << 8) | c.b;
             std::memcpy(t.color + y * t.color_stride + x * 4, &v, 4);
+            break;
+        }
+        case PF_U32_ARGB: {  // u32::from_be_bytes([a, r, g, b])
+            uint32_t v = ((uint32_t)c.a << 24) | ((uint32_t)c.r << 16) | ((uint32_t)c.g << 8) | c.b;
+            std::memcpy(t.color + y * t.color_stride + x * 4, &v, 4);
+            break;
+        }
+        case PF_U32_BGRA: {  // u32::from_be_bytes([b, g, r, a])
+            uint32_t v = ((uint32_t)c.b << 24) | ((uint32_t)c.g << 16) | ((uint32_t)c.r << 8) | c.a;
+            std::memcpy(t.color + y * t.color_stride + x * 4, &v, 4);
+            break;
+        }
+        case PF_RGBA4444: {  // (r >> 4) << 12 | (g >> 4) << 8 | (b >> 4) << 4 | a >> 4
+            uint16_t v = (uint16_t)(((c.r >> 4) << 12) | ((c.g >> 4) << 8) | ((c.b >> 4) << 4) | (c.a >> 4));
+            std::memcpy(t.color + y * t.color_stride + x * 2, &v, 2);
+            break;
+        }
+        case PF_RGB565: {  // (r >> 3 & 0x1F) << 11 | (g >> 2 & 0x3F) << 5 | (b >> 3 & 0x1F)
+            uint16_t v = (uint16_t)((((c.r >> 3) & 0x1F) << 11) | (((c.g >> 2) & 0x3F) << 5) | ((c.b >> 3) & 0x1F));
+            std::memcpy(t.color + y * t.color_stride + x * 2, &v, 2);
             break;
         }
     }

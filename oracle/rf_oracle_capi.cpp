@@ -300,7 +300,7 @@ double oracle_time_frames(int kind, const uint32_t* tris, uint64_t n_tris, const
                           uint64_t width, uint64_t height, int32_t fmt, float* depth,
                           const OracleCtx* c, const uint8_t* clear_rgba, float depth_clear,
                           int32_t reps, double* per_frame, uint64_t* stats) {
-    const uint64_t stride = width * (fmt == PF_RGB8 ? 3 : 4);
+    const uint64_t stride = width * pixfmt_bytes(fmt);
     double total = 0.0;
     for (int r = 0; r < reps; ++r) {
         auto t0 = std::chrono::steady_clock::now();

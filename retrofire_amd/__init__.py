@@ -18,9 +18,11 @@ import numpy as np
 from . import _ffi
 
 __all__ = ["Context", "Stats", "Framebuf", "Colorbuf", "RenderError", "render", "shader", "Device",
-           "default_device", "PIX_RGBA8", "PIX_RGB8", "PIX_U32_RGBA", "PIX_U32_XRGB"]
+           "default_device", "PIX_RGBA8", "PIX_RGB8", "PIX_U32_RGBA", "PIX_U32_XRGB", "PIX_U32_ARGB",
+           "PIX_U32_BGRA", "PIX_RGBA4444", "PIX_RGB565"]
 
 PIX_RGBA8, PIX_RGB8, PIX_U32_RGBA, PIX_U32_XRGB = 0, 1, 2, 3
+PIX_U32_ARGB, PIX_U32_BGRA, PIX_RGBA4444, PIX_RGB565 = 4, 5, 6, 7
 _DT = {None: 0, "less": 1, "equal": 2, "greater": 3}
 _FC = {None: 0, "back": 1, "front": 2}
 
@@ -117,14 +119,25 @@ class shader:
 
 class Colorbuf:
     """Colour-only target: `color` is (h, w, 4) uint8 (RGBA8), (h, w, 3) uint8
-    (RGB8) or (h, w) uint32 with fmt PIX_U32_RGBA / PIX_U32_XRGB."""
+    (RGB8), (h, w) uint32 with fmt PIX_U32_RGBA (the default for uint32),
+    PIX_U32_XRGB, PIX_U32_ARGB or PIX_U32_BGRA, or (h, w) uint16 with fmt
+    PIX_RGBA4444 or PIX_RGB565 (which must be given)."""
     depth = None
+
+    _ITEM = {PIX_RGBA8: (np.uint8, 3), PIX_RGB8: (np.uint8, 3), PIX_U32_RGBA: (np.uint32, 2),
+             PIX_U32_XRGB: (np.uint32, 2), PIX_U32_ARGB: (np.uint32, 2), PIX_U32_BGRA: (np.uint32, 2),
+             PIX_RGBA4444: (np.uint16, 2), PIX_RGB565: (np.uint16, 2)}
 
     def __init__(self, color: np.ndarray, fmt: int | None = None):
         self.color = color
         if fmt is None:
+            if color.dtype == np.uint16:
+                raise ValueError("a uint16 colour buffer needs fmt PIX_RGBA4444 or PIX_RGB565")
             fmt = PIX_RGB8 if color.ndim == 3 and color.shape[2] == 3 else (
                 PIX_RGBA8 if color.ndim == 3 else PIX_U32_RGBA)
+        want = self._ITEM.get(fmt)
+        if want is None or color.dtype != want[0] or color.ndim != want[1]:
+            raise ValueError(f"colour buffer of dtype {color.dtype}, {color.ndim} dims does not fit fmt {fmt}")
         self.fmt = fmt
 
 

@@ -9,7 +9,7 @@ import numpy as np
 from . import _ffi
 from . import Context, Stats, check, default_device
 
-BPP = {0: 4, 1: 3, 2: 4, 3: 4}
+BPP = {0: 4, 1: 3, 2: 4, 3: 4, 4: 4, 5: 4, 6: 2, 7: 2}
 
 
 class Mesh:
@@ -62,11 +62,12 @@ class DeviceFramebuf:
         check(_ffi.lib().rf_framebuf_create(self.dev.h, width, height, fmt, int(with_depth), C.byref(self.h)))
 
     def read(self):
-        bpp = BPP[self.fmt]
-        if self.fmt in (2, 3):
+        if self.fmt in (2, 3, 4, 5):
             col = np.empty((self.height, self.width), np.uint32)
+        elif self.fmt in (6, 7):
+            col = np.empty((self.height, self.width), np.uint16)
         else:
-            col = np.empty((self.height, self.width, bpp), np.uint8)
+            col = np.empty((self.height, self.width, BPP[self.fmt]), np.uint8)
         dep = np.empty((self.height, self.width), np.float32) if self.with_depth else None
         check(_ffi.lib().rf_framebuf_read(self.dev.h, self.h, col.ctypes.data,
                                           dep.ctypes.data_as(C.POINTER(C.c_float)) if dep is not None else None))

@@ -48,7 +48,7 @@ void DevBuf::release(uint64_t* counter) {
     cap = 0;
 }
 
-uint32_t bytes_per_pixel(uint32_t fmt) { return fmt == PIX_RGB8 ? 3u : 4u; }
+uint32_t bytes_per_pixel(uint32_t fmt) { return pixfmt_bytes(fmt); }
 
 uint32_t pack_clear(uint32_t fmt, const uint8_t* c) { return pack_pixel(fmt, c[0], c[1], c[2], c[3]); }
 
@@ -276,7 +276,7 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     if (st) return st;
     if (draw->kind > KIND_TEX_REPEAT_POT) return RF_E_UNSUPPORTED;
     if ((st = check_tex(draw->kind, draw->tex, draw->tex_w, draw->tex_h, draw->tex_stride))) return st;
-    if (!target->color || target->width == 0 || target->height == 0 || target->fmt > PIX_U32_XRGB ||
+    if (!target->color || target->width == 0 || target->height == 0 || target->fmt > PIX_LAST ||
         target->width > MAX_DIM || target->height > MAX_DIM)
         return RF_E_ARG;
     const uint32_t bpp = bytes_per_pixel(target->fmt);
@@ -470,7 +470,7 @@ rf_status rf_texture_free(rf_device* dev, rf_texture* tex) {
 
 rf_status rf_framebuf_create(rf_device* dev, uint32_t w, uint32_t h, uint32_t fmt, int with_depth,
                              rf_framebuf** out) {
-    if (!dev || !out || !w || !h || fmt > PIX_U32_XRGB || w > MAX_DIM || h > MAX_DIM) return RF_E_ARG;
+    if (!dev || !out || !w || !h || fmt > PIX_LAST || w > MAX_DIM || h > MAX_DIM) return RF_E_ARG;
     Device& d = *reinterpret_cast<Device*>(dev);
     if (hipSetDevice(d.ordinal)) return RF_E_DEVICE;
     Framebuf* f = new (std::nothrow) Framebuf();

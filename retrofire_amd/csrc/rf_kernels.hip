@@ -830,14 +830,6 @@ __device__ __forceinline__ bool depth_pass(int32_t dt, float nz, float cur) {
     }
 }
 
-__device__ __forceinline__ uint32_t load_pixel(const FrameDesc& F, uint32_t x, uint32_t y) {
-    const uint8_t* p = F.color + (uint64_t)y * F.color_stride;
-    if (F.fmt == PIX_RGB8) {
-        p += 3ull * x;
-        return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16);
-    }
-    return *(const uint32_t*)(p + 4ull * x);
-}
 __device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint32_t y, uint32_t v) {
     uint8_t* p = F.color + (uint64_t)y * F.color_stride;
     if (F.fmt == PIX_RGB8) {
@@ -845,6 +837,10 @@ __device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint
         p[0] = (uint8_t)v;
         p[1] = (uint8_t)(v >> 8);
         p[2] = (uint8_t)(v >> 16);
+        return;
+    }
+    if (F.fmt >= PIX_RGBA4444) {
+        *(uint16_t*)(p + 2ull * x) = (uint16_t)v;
         return;
     }
     *(uint32_t*)(p + 4ull * x) = v;
@@ -868,7 +864,7 @@ __device__ __forceinline__ void clear_tile(const FrameDesc& F, uint32_t tile, ui
     const uint32_t lt = tile - F.tile_base;
     const uint32_t x0 = (lt % F.ntx) * TW, y0 = (lt / F.ntx) * TILE_H;
     const uint32_t ncols = min((uint32_t)TW, F.width - x0), nrows = min((uint32_t)TILE_H, F.height - y0);
-    const bool vec = ncols == (uint32_t)TW && F.fmt != PIX_RGB8 && ((uintptr_t)F.color & 15u) == 0 &&
+    const bool vec = ncols == (uint32_t)TW && pixfmt_bytes(F.fmt) == 4u && ((uintptr_t)F.color & 15u) == 0 &&
                      (F.color_stride & 15u) == 0 &&
                      (!has_depth || (((uintptr_t)F.depth & 15u) == 0 && (F.depth_stride & 3u) == 0));
     if (vec) {
@@ -1165,7 +1161,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
         const uint32_t npx = ncols * nrows;
 
         // 16-byte stores: whole tile row of 4-byte pixels, 16-byte aligned rows
-        const bool vec = ncols == (uint32_t)TW && F.fmt != PIX_RGB8 && ((uintptr_t)F.color & 15u) == 0 &&
+        const bool vec = ncols == (uint32_t)TW && pixfmt_bytes(F.fmt) == 4u && ((uintptr_t)F.color & 15u) == 0 &&
                          (F.color_stride & 15u) == 0 && (!has_depth || (((uintptr_t)F.depth & 15u) == 0 &&
                                                                          (F.depth_stride & 3u) == 0));
         if (nruns == 0) {

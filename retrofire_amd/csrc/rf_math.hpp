@@ -92,7 +92,11 @@ RF_HD uint32_t pack_pixel(uint32_t fmt, uint32_t r, uint32_t g, uint32_t b, uint
         case 0: return r | (g << 8) | (b << 16) | (a << 24);  // [u8;4] rgba
         case 1: return r | (g << 8) | (b << 16);              // [u8;3] rgb
         case 2: return (r << 24) | (g << 16) | (b << 8) | a;  // u32 0xRRGGBBAA
-        default: return (r << 16) | (g << 8) | b;             // u32 0x00RRGGBB
+        case 3: return (r << 16) | (g << 8) | b;              // u32 0x00RRGGBB
+        case 4: return (a << 24) | (r << 16) | (g << 8) | b;  // u32 0xAARRGGBB
+        case 5: return (b << 24) | (g << 16) | (r << 8) | a;  // u32 0xBBGGRRAA
+        case 6: return ((r >> 4) << 12) | ((g >> 4) << 8) | ((b >> 4) << 4) | (a >> 4);  // u16 4444
+        default: return (((r >> 3) & 0x1F) << 11) | (((g >> 2) & 0x3F) << 5) | ((b >> 3) & 0x1F);  // u16 565
     }
 }
 

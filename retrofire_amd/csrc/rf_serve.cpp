@@ -468,7 +468,7 @@ rf_status rf_serve_open(rf_device* dev, uint64_t budget_bytes, uint32_t max_batc
 }
 
 rf_status rf_serve_submit(rf_serve* sh, const rf_request* req) {
-    if (!sh || !req || req->width == 0 || req->height == 0 || req->fmt > PIX_U32_XRGB || (req->n_draws && !req->draws) ||
+    if (!sh || !req || req->width == 0 || req->height == 0 || req->fmt > PIX_LAST || (req->n_draws && !req->draws) ||
         req->width > MAX_DIM || req->height > MAX_DIM)
         return RF_E_ARG;
     Serve& s = *reinterpret_cast<Serve*>(sh);

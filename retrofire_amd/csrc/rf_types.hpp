@@ -43,7 +43,15 @@ enum PixFmt : uint32_t {
     PIX_RGB8 = 1,       // [u8; 3] r, g, b     (Buf2<Color3>)
     PIX_U32_RGBA = 2,   // u32 0xRRGGBBAA      (Rgba8888 into u32)
     PIX_U32_XRGB = 3,   // u32 0x00RRGGBB      (Xrgb8888)
+    PIX_U32_ARGB = 4,   // u32 0xAARRGGBB      (Argb8888 into u32; Bgra8888 bytes)
+    PIX_U32_BGRA = 5,   // u32 0xBBGGRRAA      (Bgra8888 into u32; Argb8888 bytes)
+    PIX_RGBA4444 = 6,   // u16 rrrrggggbbbbaaaa (Rgba4444)
+    PIX_RGB565 = 7,     // u16 rrrrrggggggbbbbb (Rgb565)
 };
+constexpr uint32_t PIX_LAST = PIX_RGB565;
+__host__ __device__ inline uint32_t pixfmt_bytes(uint32_t fmt) {
+    return fmt == PIX_RGB8 ? 3u : (fmt >= PIX_RGBA4444 ? 2u : 4u);
+}
 
 enum DepthTest : int32_t { DT_NONE = 0, DT_LESS = 1, DT_EQUAL = 2, DT_GREATER = 3 };
 enum FaceCull : int32_t { FC_NONE = 0, FC_BACK = 1, FC_FRONT = 2 };

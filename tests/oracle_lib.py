@@ -72,7 +72,7 @@ def lib():
     return _lib
 
 
-FMT_BPP = {0: 4, 1: 3, 2: 4, 3: 4}
+FMT_BPP = {0: 4, 1: 3, 2: 4, 3: 4, 4: 4, 5: 4, 6: 2, 7: 2}
 
 
 def render(kind, tris, pos, attr, mvp, vp, color, depth=None, fmt=0, flat=(255, 255, 255, 255),

@@ -11,8 +11,10 @@ FC = {None: 0, "back": 1, "front": 2}
 
 def make_target(w, h, fmt, with_depth, seed=0, clear=None):
     rng = np.random.default_rng(seed)
-    if fmt in (rf.PIX_U32_RGBA, rf.PIX_U32_XRGB):
+    if fmt in (rf.PIX_U32_RGBA, rf.PIX_U32_XRGB, rf.PIX_U32_ARGB, rf.PIX_U32_BGRA):
         col = rng.integers(0, 2 ** 32, size=(h, w), dtype=np.uint32)
+    elif fmt in (rf.PIX_RGBA4444, rf.PIX_RGB565):
+        col = rng.integers(0, 2 ** 16, size=(h, w), dtype=np.uint16)
     else:
         col = rng.integers(0, 256, size=(h, w, 3 if fmt == rf.PIX_RGB8 else 4), dtype=np.uint8)
     dep = None
@@ -29,6 +31,14 @@ def pack_clear(fmt, c):
         return np.array([r, g, b], np.uint8)
     if fmt == rf.PIX_U32_RGBA:
         return np.uint32((r << 24) | (g << 16) | (b << 8) | a)
+    if fmt == rf.PIX_U32_ARGB:
+        return np.uint32((a << 24) | (r << 16) | (g << 8) | b)
+    if fmt == rf.PIX_U32_BGRA:
+        return np.uint32((b << 24) | (g << 16) | (r << 8) | a)
+    if fmt == rf.PIX_RGBA4444:
+        return np.uint16(((r >> 4) << 12) | ((g >> 4) << 8) | ((b >> 4) << 4) | (a >> 4))
+    if fmt == rf.PIX_RGB565:
+        return np.uint16(((r >> 3) << 11) | ((g >> 2) << 5) | (b >> 3))
     return np.uint32((r << 16) | (g << 8) | b)
 
 

@@ -73,7 +73,8 @@ def test_icospheres_parity_reduced():
     assert g[2][6] > 10000
 
 
-@pytest.mark.parametrize("fmt", [rf.PIX_RGBA8, rf.PIX_RGB8, rf.PIX_U32_RGBA, rf.PIX_U32_XRGB])
+@pytest.mark.parametrize("fmt", [rf.PIX_RGBA8, rf.PIX_RGB8, rf.PIX_U32_RGBA, rf.PIX_U32_XRGB, rf.PIX_U32_ARGB,
+                                 rf.PIX_U32_BGRA, rf.PIX_RGBA4444, rf.PIX_RGB565])
 def test_pixel_formats(fmt):
     _scene_parity(S.sphere640(), fmt=fmt, ctx=rf.Context(clear=True))
     _scene_parity(S.sphere640(), fmt=fmt, ctx=rf.Context(clear=False), with_depth=False)

@@ -353,6 +353,59 @@ def test_hello_tri_fp_golden_coverage():  # core/triangle.ppm, hello_tri.rs `fp`
     assert diff.max() <= 1  # colours go through libm powf: +-1 LSB
 
 
+# -------------------------------------------------------------- pixfmt.rs
+
+def _flat_fill(fmt, color, shape, dtype):
+    """Fills a 4x2 target of pixel format `fmt` with a flat-shaded quad."""
+    w, h = 4, 2
+    verts = np.array([(-1, -1, 0), (1, -1, 0), (-1, 1, 0), (1, 1, 0)], np.float32)
+    faces = np.array([[0, 2, 1], [1, 2, 3]], np.uint32)
+    fb = np.zeros((h, w) + shape, dtype)
+    err, st = O.render(0, faces, verts, None, M.identity(), M.viewport(0, 0, w, h), fb, fmt=fmt,
+                       flat=color, ctx=(0, 0, 1, 1))
+    assert err == 0 and int(st[6]) == w * h
+    return fb
+
+
+@pytest.mark.parametrize("fmt,color,shape,dtype,want", [
+    (0, (0x11, 0x22, 0x33, 0x44), (4,), np.uint8, [0x11, 0x22, 0x33, 0x44]),  # pixfmt.rs:175-184
+    (2, (0x11, 0x22, 0x33, 0x44), (), np.uint32, 0x11_22_33_44),               # pixfmt.rs:175-181
+    (4, (0x11, 0x22, 0x33, 0x44), (), np.uint32, 0x44_11_22_33),               # pixfmt.rs:188-195
+    (5, (0x11, 0x22, 0x33, 0x44), (), np.uint32, 0x33_22_11_44),               # pixfmt.rs:197-204
+    (6, (0x11, 0x22, 0x33, 0x44), (), np.uint16, 0x1234),                      # pixfmt.rs:206-213
+    (7, (0x40, 0x20, 0x10, 0x99), (), np.uint16, 0b01000_001000_00010),        # pixfmt.rs:157-164
+    (1, (0x11, 0x22, 0x33, 0x44), (3,), np.uint8, [0x11, 0x22, 0x33]),
+    (3, (0x11, 0x22, 0x33, 0x44), (), np.uint32, 0x00_11_22_33),               # pixfmt.rs:150-154
+])
+def test_pixel_format_known_answers(fmt, color, shape, dtype, want):
+    fb = _flat_fill(fmt, color, shape, dtype)
+    assert (fb == np.array(want, dtype)).all(), fb
+
+
+def test_pixel_format_byte_orders():
+    # The byte forms ([u8; 4] Argb8888 = [a, r, g, b], [u8; 4] Bgra8888 =
+    # [b, g, r, a], [u8; 2] Rgba4444 = [0x34, 0x12], [u8; 2] Rgb565 =
+    # [0b000_00010, 0b01000_001]; pixfmt.rs:157-213) are the little-endian
+    # bytes of the u32 / u16 forms of the swapped or same format.
+    c = (0x11, 0x22, 0x33, 0x44)
+    assert _flat_fill(5, c, (), np.uint32).view(np.uint8)[0, :4].tolist() == [0x44, 0x11, 0x22, 0x33]
+    assert _flat_fill(4, c, (), np.uint32).view(np.uint8)[0, :4].tolist() == [0x33, 0x22, 0x11, 0x44]
+    assert _flat_fill(6, c, (), np.uint16).view(np.uint8)[0, :2].tolist() == [0x34, 0x12]
+    assert _flat_fill(7, (0x40, 0x20, 0x10, 0), (), np.uint16).view(np.uint8)[0, :2].tolist() == \
+        [0b000_00010, 0b01000_001]
+
+
+def test_rgb565_and_4444_drop_low_bits():
+    # Brute force over every channel value: 565 keeps the top 5/6/5 bits,
+    # 4444 the top 4 of each channel.
+    for v in range(256):
+        p565 = int(_flat_fill(7, (v, v, v, 0), (), np.uint16)[0, 0])
+        assert p565 >> 11 == v >> 3 and (p565 >> 5) & 0x3F == v >> 2 and p565 & 0x1F == v >> 3
+        if v % 17 == 0:
+            p4 = int(_flat_fill(6, (v, 255 - v, v, 255 - v), (), np.uint16)[0, 0])
+            assert p4 == ((v >> 4) << 12) | (((255 - v) >> 4) << 8) | ((v >> 4) << 4) | ((255 - v) >> 4)
+
+
 # ------------------------------------------- closed-form and brute-force pins
 
 def test_axis_aligned_rect_exact_coverage():
