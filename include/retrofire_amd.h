@@ -95,6 +95,10 @@ typedef struct rf_stats {
     uint64_t verts_i, verts_o;
     uint64_t frags_i, frags_o;
     double seconds;
+    /* Stats::objs (stats.rs): draws that asked for the bounding-box test
+     * (rf_mesh_draw.cull) and those of them that it kept. rf_render leaves
+     * them as they are, as render() does. */
+    uint64_t objs_i, objs_o;
 } rf_stats;
 
 /* A render target: a colour buffer and an optional f32 depth buffer holding
@@ -184,6 +188,11 @@ typedef struct rf_mesh_draw {
     const rf_texture* tex;
     uint64_t first_tri;      /* draw triangles [first_tri, first_tri + n_tris) */
     uint64_t n_tris;         /* 0: to the end of the mesh */
+    /* 1: whole-object culling on the device, as callers do before render()
+     * (crates.rs:118-122): the draw is skipped, and counts in no stats but
+     * objs_i, when BBox::visibility(mvp) is Hidden for the mesh's bounding
+     * box (scene.rs:37-39, 81-87). 0: always drawn. */
+    uint32_t cull;
 } rf_mesh_draw;
 
 typedef struct rf_frame {

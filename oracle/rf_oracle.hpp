@@ -291,6 +291,42 @@ inline void apply_proj(const float* m, const float* p3, float* out4) {
     out4[3] = dot4(m + 12, h);
 }
 
+// ---------------------------------------------------------------------------
+// Whole-object culling by the caller (scene.rs, crates.rs:114-133)
+// ---------------------------------------------------------------------------
+// BBox::of (scene.rs:37-39): BBox::default() = (+inf, -inf) (scene.rs:107-112)
+// extended by every position (scene.rs:42-46, 115-119) with f32::min / f32::max,
+// which return the other operand when one is NaN (as fminf / fmaxf do).
+inline void bbox_of(const float* pos, uint64_t n, float* bb6) {
+    for (int k = 0; k < 3; ++k) {
+        bb6[k] = INFINITY;
+        bb6[3 + k] = -INFINITY;
+    }
+    for (uint64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            bb6[k] = std::fmin(bb6[k], pos[3 * i + k]);
+            bb6[3 + k] = std::fmax(bb6[3 + k], pos[3 * i + k]);
+        }
+}
+
+// BBox::visibility (scene.rs:81-87): the 8 corners of BBox::verts
+// (scene.rs:59-66) through ProjMat3::apply, as ClipVerts, then
+// view_frustum::status.
+inline Status bbox_visibility(const float* bb6, const float* mvp) {
+    const float x[2] = {bb6[0], bb6[3]}, y[2] = {bb6[1], bb6[4]}, z[2] = {bb6[2], bb6[5]};
+    ClipVert<0> vs[8];
+    int n = 0;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+                const float p[3] = {x[a], y[b], z[c]};
+                float clip[4];
+                apply_proj(mvp, p, clip);
+                vs[n++] = clip_vert<0>(clip, Attr<0>{});
+            }
+    return status<0>(vs, 8);
+}
+
 // to_screen (prim.rs:62-88): pt3(x, y, 1).z_div(w) is (x/w, y/w, 1/w); the
 // attribute is z_div'ed by w componentwise when the varying type divides
 // (vec.rs:666-676 for Vec2/TexCoord, vary.rs:135-140 for f32), and left as is

@@ -196,6 +196,11 @@ void oracle_scanline_frags_f32(const float* a, const float* b, uint32_t n, float
     }
 }
 
+// BBox::of over n positions (x, y, z) into bb6 = low xyz, high xyz.
+void oracle_bbox_of(const float* pos, uint64_t n, float* bb6) { bbox_of(pos, n, bb6); }
+// BBox::visibility: 0 Visible, 1 Clipped, 2 Hidden (clip.rs:65-72).
+int oracle_bbox_visibility(const float* bb6, const float* mvp) { return (int)bbox_visibility(bb6, mvp); }
+
 float oracle_signed_dist(int plane, const float* pos) { return signed_dist(plane, pos); }
 int oracle_outcode(const float* pos) { return outcode(pos); }
 

@@ -49,9 +49,11 @@ class Stats:
     frags_i: int = 0
     frags_o: int = 0
     seconds: float = 0.0
+    objs_i: int = 0
+    objs_o: int = 0
 
     def add(self, s: "_ffi.RfStats"):
-        for k in ("calls", "prims_i", "prims_o", "verts_i", "verts_o", "frags_i", "frags_o"):
+        for k in ("calls", "prims_i", "prims_o", "verts_i", "verts_o", "frags_i", "frags_o", "objs_i", "objs_o"):
             setattr(self, k, getattr(self, k) + int(getattr(s, k)))
         self.seconds += float(s.seconds)
 

@@ -22,7 +22,8 @@ class RfCtx(C.Structure):
 class RfStats(C.Structure):
     _fields_ = [("calls", C.c_uint64), ("prims_i", C.c_uint64), ("prims_o", C.c_uint64),
                 ("verts_i", C.c_uint64), ("verts_o", C.c_uint64), ("frags_i", C.c_uint64),
-                ("frags_o", C.c_uint64), ("seconds", C.c_double)]
+                ("frags_o", C.c_uint64), ("seconds", C.c_double), ("objs_i", C.c_uint64),
+                ("objs_o", C.c_uint64)]
 
 
 class RfTarget(C.Structure):
@@ -42,7 +43,7 @@ class RfDraw(C.Structure):
 class RfMeshDraw(C.Structure):
     _fields_ = [("mesh", C.c_void_p), ("kind", C.c_uint32), ("mvp", C.c_float * 16),
                 ("viewport", C.c_float * 16), ("flat_color", C.c_uint8 * 4), ("tex", C.c_void_p),
-                ("first_tri", C.c_uint64), ("n_tris", C.c_uint64)]
+                ("first_tri", C.c_uint64), ("n_tris", C.c_uint64), ("cull", C.c_uint32)]
 
 
 class RfFrame(C.Structure):

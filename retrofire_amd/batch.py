@@ -79,9 +79,12 @@ class DeviceFramebuf:
             self.h = C.c_void_p()
 
 
-def mesh_draw(mesh: Mesh, shd, mvp, viewport, first_tri=0, n_tris=0, texture: Texture | None = None):
+def mesh_draw(mesh: Mesh, shd, mvp, viewport, first_tri=0, n_tris=0, texture: Texture | None = None,
+              cull=False):
     """One render() call over a resident mesh: `shd` is a shader kind object
-    (retrofire_amd.shader.*); textured kinds take a resident `texture`."""
+    (retrofire_amd.shader.*); textured kinds take a resident `texture`. With
+    `cull`, the device skips the draw when the mesh's bounding box is outside
+    the view frustum (BBox::visibility is Hidden, scene.rs:81-87)."""
     d = _ffi.RfMeshDraw()
     d.mesh = mesh.h.value
     d.kind = shd.kind
@@ -98,6 +101,7 @@ def mesh_draw(mesh: Mesh, shd, mvp, viewport, first_tri=0, n_tris=0, texture: Te
         d.tex = texture.h.value
     d.first_tri = first_tri
     d.n_tris = n_tris
+    d.cull = int(bool(cull))
     return d
 
 

@@ -119,6 +119,7 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
     if ((e = runs.ensure(std::max<uint64_t>(B.runs_cap, 1) * sizeof(Run), counter))) return e;
     if ((e = misc.ensure(256, counter))) return e;
     if ((e = stats.ensure(std::max<uint64_t>(B.n_frames, 1) * sizeof(FrameStats), counter))) return e;
+    if ((e = draw_vis.ensure(std::max<uint64_t>(B.n_draws, 1) * sizeof(uint32_t), counter))) return e;
     B.cv = (float4*)cv.p;
     B.ca = (float4*)ca.p;
     B.sv = (float*)sv.p;
@@ -140,12 +141,13 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
     B.tile_counter = (uint32_t*)misc.p + 16;  // TILE_PARTS + 1 words (busy partitions, empty tiles)
     B.err = (uint32_t*)misc.p + 2;
     B.stats = (FrameStats*)stats.p;
+    B.draw_vis = (uint32_t*)draw_vis.p;
     return hipSuccess;
 }
 
 void Scratch::release(uint64_t* counter) {
     for (DevBuf* b : {&cv, &ca, &sv, &hdr, &scr, &extra, &rect, &run_count, &run_off, &run_cursor,
-                      &scan_blk, &tile_frame, &tile_list, &group_draw, &vblock_draw, &runs, &misc, &stats})
+                      &scan_blk, &tile_frame, &tile_list, &group_draw, &vblock_draw, &runs, &misc, &stats, &draw_vis})
         b->release(counter);
 }
 
@@ -424,6 +426,17 @@ rf_status rf_mesh_create(rf_device* dev, const float* pos, const float* attr, ui
         rf_mesh_free(dev, reinterpret_cast<rf_mesh*>(m));
         return RF_E_DEVICE;
     }
+    // BBox::of (scene.rs:37-45, 110-114): from (+inf, -inf), extended by each
+    // position with f32::min / f32::max, which skip a NaN operand like fminf / fmaxf
+    for (int k = 0; k < 3; ++k) {
+        m->bbox[k] = INFINITY;
+        m->bbox[3 + k] = -INFINITY;
+    }
+    for (uint64_t i = 0; i < n_verts; ++i)
+        for (int k = 0; k < 3; ++k) {
+            m->bbox[k] = std::fmin(m->bbox[k], pos[3 * i + k]);
+            m->bbox[3 + k] = std::fmax(m->bbox[3 + k], pos[3 * i + k]);
+        }
     *out = reinterpret_cast<rf_mesh*>(m);
     return RF_OK;
 }

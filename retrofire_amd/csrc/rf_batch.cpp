@@ -24,7 +24,7 @@ struct Batch {
     std::vector<std::vector<rf_mesh_draw>> draws;
     std::vector<DrawDesc> h_draws;
     std::vector<FrameDesc> h_frames;
-    std::vector<uint64_t> frame_prims_i, frame_verts_i, frame_calls;
+    std::vector<uint64_t> frame_prims_i, frame_verts_i, frame_calls, frame_objs;
     DevBuf d_desc;
     Scratch sc;
     BatchDesc B{};
@@ -44,6 +44,8 @@ static rf_status build_descs(Batch& b, uint32_t tile_w) {
     b.frame_prims_i.assign(b.frames.size(), 0);
     b.frame_verts_i.assign(b.frames.size(), 0);
     b.frame_calls.assign(b.frames.size(), 0);
+    b.frame_objs.assign(b.frames.size(), 0);
+    uint32_t any_cull = 0;
     uint64_t vbase = 0, tbase = 0;
     uint32_t tile_base = 0;
     for (size_t fi = 0; fi < b.frames.size(); ++fi) {
@@ -86,11 +88,15 @@ static rf_status build_descs(Batch& b, uint32_t tile_w) {
             dd.tri_base = tbase;
             std::memcpy(dd.mvp, md.mvp, 64);
             std::memcpy(dd.vp, md.viewport, 64);
+            dd.cull = md.cull;
+            std::memcpy(dd.bbox, m->bbox, sizeof(dd.bbox));
+            any_cull |= md.cull;
             vbase += m->n_verts;
             tbase += nt;
             b.frame_prims_i[fi] += nt;
             b.frame_verts_i[fi] += m->n_verts;
             b.frame_calls[fi] += 1;
+            b.frame_objs[fi] += md.cull;
             b.h_draws.push_back(dd);
         }
     }
@@ -109,6 +115,7 @@ static rf_status build_descs(Batch& b, uint32_t tile_w) {
     B.n_tris = tbase;
     B.n_groups = (uint32_t)((tbase + WAVE_SIZE - 1) / WAVE_SIZE);
     B.n_tiles = tile_base;
+    B.any_cull = any_cull;
     B.tile_w = tile_w;
     B.tile_h = TILE_ROWS;
     B.extra_cap = std::max<uint32_t>(b.B.extra_cap, 1024u);
@@ -189,7 +196,7 @@ rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_fra
         if (st) return st;
         for (uint32_t k = 0; k < f.n_draws; ++k) {
             const rf_mesh_draw& md = f.draws[k];
-            if (!md.mesh || md.kind > KIND_TEX_REPEAT_POT) return RF_E_ARG;
+            if (!md.mesh || md.kind > KIND_TEX_REPEAT_POT || md.cull > 1) return RF_E_ARG;
             const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
             const Tex* t = reinterpret_cast<const Tex*>(md.tex);
             if ((int)m->n_attr < kind_attrs(md.kind)) return RF_E_ARG;
@@ -278,9 +285,12 @@ rf_status rf_batch_wait(rf_batch* bh, rf_stats* stats) {
         (void)hipEventElapsedTime(&ms, b.ev0, b.ev1);
         for (uint32_t i = 0; i < b.B.n_frames; ++i) {
             rf_stats& s = stats[i];
-            s.calls = b.frame_calls[i];
-            s.prims_i = b.frame_prims_i[i];
-            s.verts_i = b.frame_verts_i[i];
+            const FrameStats& h = b.h_stats[i];
+            s.calls = b.frame_calls[i] - h.hidden_draws;
+            s.prims_i = b.frame_prims_i[i] - h.hidden_prims;
+            s.verts_i = b.frame_verts_i[i] - h.hidden_verts;
+            s.objs_i = b.frame_objs[i];
+            s.objs_o = b.frame_objs[i] - h.hidden_draws;
             s.prims_o = b.h_stats[i].prims_o;
             s.verts_o = b.h_stats[i].verts_o;
             s.frags_i = b.h_stats[i].frags_i;

@@ -32,7 +32,7 @@ struct DevBuf {
 // Per-pipeline scratch buffers.
 struct Scratch {
     DevBuf cv, ca, sv, hdr, scr, extra, rect, run_count, run_off, run_cursor, scan_blk, tile_frame, tile_list, group_draw, vblock_draw, runs, misc,
-        stats;
+        stats, draw_vis;
     hipError_t prepare(BatchDesc& B, uint64_t* counter);
     void release(uint64_t* counter);
 };
@@ -52,6 +52,7 @@ struct Mesh {
     DevBuf pos, attr, tris;
     uint64_t n_verts = 0, n_tris = 0;
     uint32_t n_attr = 0;
+    float bbox[6] = {0, 0, 0, 0, 0, 0};  // BBox::of over the positions (scene.rs:37-39)
 };
 
 struct Tex {

@@ -146,6 +146,7 @@ __global__ __launch_bounds__(256) void k_vertex(BatchDesc B) {
     const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= B.n_verts) return;
     const uint32_t di = draw_of_vert(B.draws, B.n_draws, B.vblock_draw[slot / 256], slot);
+    if (B.any_cull && !B.draw_vis[di]) return;  // a draw the bounding-box test hid: nothing reads these slots
     const DrawDesc* D = &B.draws[di];
     const uint64_t i = slot - D->vert_base;
     const float px = D->pos[3 * i + 0], py = D->pos[3 * i + 1], pz = D->pos[3 * i + 2];
@@ -354,7 +355,9 @@ __global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
         frame = D->frame;
         const uint64_t t = g - D->tri_base;
         const uint32_t i0 = D->tris[3 * t], i1 = D->tris[3 * t + 1], i2 = D->tris[3 * t + 2];
-        if (i0 >= D->n_verts || i1 >= D->n_verts || i2 >= D->n_verts) {
+        if (B.any_cull && !B.draw_vis[di]) {
+            // the caller skips a hidden object before render() (crates.rs:118-122)
+        } else if (i0 >= D->n_verts || i1 >= D->n_verts || i2 >= D->n_verts) {
             atomicOr(B.err, ERRB_INDEX);
         } else {
             const uint64_t s0 = D->vert_base + i0, s1 = D->vert_base + i1, s2 = D->vert_base + i2;
@@ -1641,6 +1644,40 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_cull: one thread per draw. A draw that asks for it is skipped when its
+// mesh's bounding box is Hidden: the 8 corners (BBox::verts, scene.rs:59-66)
+// through the draw's ProjMat3 (scene.rs:81-87) and view_frustum::status
+// (clip.rs:245-267) — all corners outside one same plane. What the skipped
+// render() call would have counted is added to the frame's hidden totals.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cull(BatchDesc B) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n_draws) return;
+    const DrawDesc* D = &B.draws[i];
+    uint32_t vis = 1;
+    if (D->cull) {
+        const float* m = D->mvp;
+        uint32_t all_out = ~0u;
+        for (int c = 0; c < 8; ++c) {
+            const float px = D->bbox[(c & 4) ? 3 : 0], py = D->bbox[(c & 2) ? 4 : 1], pz = D->bbox[(c & 1) ? 5 : 2];
+            const float x = dot4(m + 0, px, py, pz, 1.0f);
+            const float y = dot4(m + 4, px, py, pz, 1.0f);
+            const float z = dot4(m + 8, px, py, pz, 1.0f);
+            const float w = dot4(m + 12, px, py, pz, 1.0f);
+            all_out &= outcode(x, y, z, w);
+        }
+        if (all_out != 0) {
+            vis = 0;
+            FrameStats& st = B.stats[D->frame];
+            atomicAdd(&st.hidden_draws, 1ull);
+            atomicAdd(&st.hidden_prims, (unsigned long long)D->n_tris);
+            atomicAdd(&st.hidden_verts, (unsigned long long)D->n_verts);
+        }
+    }
+    B.draw_vis[i] = vis;
+}
+
 // Zeroes per-batch counters (a kernel, so it can live inside a captured graph).
 __global__ void k_reset(BatchDesc B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1653,6 +1690,9 @@ __global__ void k_reset(BatchDesc B) {
         B.stats[i].verts_o = 0;
         B.stats[i].frags_i = 0;
         B.stats[i].frags_o = 0;
+        B.stats[i].hidden_draws = 0;
+        B.stats[i].hidden_prims = 0;
+        B.stats[i].hidden_verts = 0;
     }
     if (i == 0) {
         *B.extra_count = 0;
@@ -1682,6 +1722,7 @@ static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raste
     };
     mark(0);
     hipLaunchKernelGGL(k_reset, dim3((reset_n + 255) / 256 + 1), dim3(256), 0, s, B);
+    if (B.any_cull) hipLaunchKernelGGL(k_cull, dim3((B.n_draws + 255) / 256), dim3(256), 0, s, B);
     if (B.n_verts)
         hipLaunchKernelGGL(k_vertex, dim3((uint32_t)((B.n_verts + 255) / 256)), dim3(256), 0, s, B);
     mark(1);

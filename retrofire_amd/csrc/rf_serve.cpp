@@ -190,10 +190,18 @@ static void finish_lane(Serve& s, Lane& l, rf_status st) {
         d.res.submit_s = q.submit_s;
         d.res.start_s = l.start_s;
         d.res.done_s = l.done_s;
+        uint64_t objs = 0;
+        for (const rf_mesh_draw& md : q.draws) objs += md.cull;
         d.res.stats.calls = q.draws.size();
         d.res.stats.prims_i = q.n_tris;
         d.res.stats.verts_i = q.n_verts;
+        d.res.stats.objs_i = d.res.stats.objs_o = objs;
         if (st == RF_OK) {
+            const FrameStats& h = l.h_stats[i];
+            d.res.stats.calls -= h.hidden_draws;
+            d.res.stats.prims_i -= h.hidden_prims;
+            d.res.stats.verts_i -= h.hidden_verts;
+            d.res.stats.objs_o -= h.hidden_draws;
             d.res.stats.prims_o = l.h_stats[i].prims_o;
             d.res.stats.verts_o = l.h_stats[i].verts_o;
             d.res.stats.frags_i = l.h_stats[i].frags_i;
@@ -223,6 +231,7 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     uint64_t vbase = 0, tbase = 0;
     uint32_t tile_base = 0;
     const uint32_t tile_w = 64;
+    uint32_t any_cull = 0;
     for (size_t i = 0; i < l.reqs.size(); ++i) {
         Req& q = l.reqs[i];
         Framebuf* fb = l.fbs[i];
@@ -265,6 +274,9 @@ static rf_status launch_lane(Serve& s, Lane& l) {
             dd.tri_base = tbase;
             std::memcpy(dd.mvp, md.mvp, 64);
             std::memcpy(dd.vp, md.viewport, 64);
+            dd.cull = md.cull;
+            std::memcpy(dd.bbox, m->bbox, sizeof(dd.bbox));
+            any_cull |= md.cull;
             vbase += m->n_verts;
             tbase += nt;
             dds.push_back(dd);
@@ -298,6 +310,7 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     B.n_verts = vbase;
     B.n_tris = tbase;
     B.n_groups = (uint32_t)((tbase + WAVE_SIZE - 1) / WAVE_SIZE);
+    B.any_cull = any_cull;
     B.n_tiles = tile_base;
     B.tile_w = tile_w;
     B.tile_h = TILE_ROWS;
@@ -479,7 +492,7 @@ rf_status rf_serve_submit(rf_serve* sh, const rf_request* req) {
     q.draws.assign(req->draws, req->draws + req->n_draws);
     q.r.draws = nullptr;
     for (const rf_mesh_draw& md : q.draws) {
-        if (!md.mesh || md.kind > KIND_TEX_REPEAT_POT) return RF_E_ARG;
+        if (!md.mesh || md.kind > KIND_TEX_REPEAT_POT || md.cull > 1) return RF_E_ARG;
         const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
         const Tex* t = reinterpret_cast<const Tex*>(md.tex);
         if ((int)m->n_attr < kind_attrs(md.kind)) return RF_E_ARG;

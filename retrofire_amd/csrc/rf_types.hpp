@@ -78,11 +78,12 @@ struct DrawDesc {
     uint32_t tex_w, tex_h, tex_stride;  // stride in texels
     uint32_t flat_rgba;                 // r | g << 8 | b << 16 | a << 24
     uint32_t attr_stride;               // floats per vertex in `attr`
-    uint32_t pad0;
+    uint32_t cull;                      // 1: skipped when BBox::visibility(mvp) is Hidden (scene.rs:81-87)
     uint64_t vert_base;                 // first slot in the batch vertex buffers
     uint64_t tri_base;                  // first global triangle index
     float mvp[16];                      // row-major ProjMat3 (the uniform)
     float vp[16];                       // row-major Mat4<Ndc, Screen>
+    float bbox[6];                      // the mesh's BBox::of (scene.rs:37-39): low x, y, z, high x, y, z
 };
 
 struct FrameDesc {
@@ -103,6 +104,7 @@ struct FrameDesc {
 // Per-frame statistics (stats.rs:16-40), accumulated by the kernels.
 struct FrameStats {
     unsigned long long prims_o, verts_o, frags_i, frags_o;
+    unsigned long long hidden_draws, hidden_prims, hidden_verts;  // draws skipped by the bounding-box test
 };
 
 // Screen-space triangle as stored by the setup kernel: three vertices of
@@ -155,6 +157,8 @@ struct BatchDesc {
     uint32_t runs_cap;
     uint32_t* tile_counter;  // persistent raster work counters, one per tile partition (TILE_PARTS)
     uint32_t* err;
+    uint32_t* draw_vis;  // per draw: 0 if the bounding-box test hid it, else 1
+    uint32_t any_cull;   // some draw asks for the bounding-box test
 };
 
 }  // namespace rf

@@ -260,6 +260,22 @@ def crates_world(n_crates: int = 1000, seed: int = DEFAULT_SEED + 1):
     return np.concatenate(pos), np.concatenate(uv), np.concatenate(tris), tex
 
 
+def crates_objects(n_crates: int = 1000, seed: int = DEFAULT_SEED + 1):
+    """The crates of crates_world as separate objects, as the crates demo
+    keeps them (crates.rs:114-133, Obj scene.rs:14-18): one cube Mesh<TexCoord>
+    and a model-to-world matrix per crate."""
+    rng = Xorshift64(seed)
+    cpos, cuv, ctri = cube(1.0)
+    models = []
+    for i in range(n_crates):
+        x = rng.uniform_f32(-50.0, 50.0)
+        z = rng.uniform_f32(-50.0, 50.0)
+        yaw = rng.uniform_f32(0.0, 1.0)
+        models.append(M.then(M.rotate_y(float(yaw)), M.translate3(x, 0.0, z)))
+    tex = crate_texture(rng)
+    return cpos, cuv, ctri, models, tex
+
+
 def first_person(x: float, y: float, z: float, yaw_turns: float = 0.0) -> np.ndarray:
     """World-to-view of a level first-person camera at (x, y, z) turned by
     `yaw_turns` about +y (heading +z at yaw 0): the inverse camera transform

@@ -48,6 +48,10 @@ def lib():
         L.oracle_vary_f32.argtypes = [C.c_float, C.c_float, C.c_uint32, f32p]
         L.oracle_vary_to_f32.argtypes = [C.c_float, C.c_float, C.c_uint32, f32p]
         L.oracle_scanline_frags_f32.argtypes = [f32p, f32p, C.c_uint32, f32p]
+        L.oracle_bbox_of.restype = None
+        L.oracle_bbox_of.argtypes = [f32p, C.c_uint64, f32p]
+        L.oracle_bbox_visibility.restype = C.c_int
+        L.oracle_bbox_visibility.argtypes = [f32p, f32p]
         L.oracle_signed_dist.restype = C.c_float
         L.oracle_signed_dist.argtypes = [C.c_int, f32p]
         L.oracle_outcode.restype = C.c_int
@@ -104,3 +108,21 @@ def render(kind, tris, pos, attr, mvp, vp, color, depth=None, fmt=0, flat=(255, 
                           _p(depth, C.c_float), (w if depth is not None else 0), C.byref(c),
                           _p(stats, C.c_uint64))
     return err, stats
+
+
+VISIBLE, CLIPPED, HIDDEN = 0, 1, 2
+
+
+def bbox_of(pos):
+    """BBox::of (scene.rs:37-39) of an (n, 3) float32 array: (low xyz, high xyz)."""
+    p = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+    out = np.zeros(6, np.float32)
+    lib().oracle_bbox_of(_p(p, C.c_float), p.size // 3, _p(out, C.c_float))
+    return out
+
+
+def bbox_visibility(bb6, mvp):
+    """BBox::visibility (scene.rs:81-87): VISIBLE, CLIPPED or HIDDEN."""
+    b = np.ascontiguousarray(bb6, dtype=np.float32).reshape(6)
+    m = np.ascontiguousarray(mvp, dtype=np.float32).reshape(16)
+    return lib().oracle_bbox_visibility(_p(b, C.c_float), _p(m, C.c_float))

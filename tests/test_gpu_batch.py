@@ -160,3 +160,69 @@ def test_serving_budget_rejects_oversized():
     assert len(res) == 1 and res[0].id == 7 and res[0].status == -9
     srv.close()
     m.free()
+
+
+def test_object_culling_matches_caller_side_culling():
+    """Per-object draws with `cull` set: the device skips each object whose
+    bounding box is Hidden, exactly as the crates demo skips it before calling
+    render() (crates.rs:114-133). The oracle side does that culling itself,
+    object by object, and renders the rest in order; images and every stat
+    (objs included) agree. The first 20 draws ask for no culling."""
+    cpos, cuv, ctri, models, tex_px = S.crates_objects(300)
+    cube = RB.Mesh(cpos, ctri, cuv)
+    tex = RB.Texture(tex_px)
+    shd = rf.shader.TexRepeatPot(tex_px)
+    bb = O.bbox_of(cpos)
+    frames = []
+    for k, (w, h, yaw) in enumerate([(1280, 720, 0.0), (640, 480, 0.3), (1920, 1080, 0.62)]):
+        wtp = M.then(S.first_person(0.0, 1.7, 0.0, yaw), _persp(w, h))
+        frames.append(dict(w=w, h=h, vp=M.viewport(0, h, w, 0), mvps=[M.then(m, wtp) for m in models]))
+    fbs, spec = [], []
+    for f in frames:
+        fb = RB.DeviceFramebuf(f["w"], f["h"], 0, True)
+        fbs.append(fb)
+        draws = [RB.mesh_draw(cube, shd, mvp, f["vp"], texture=tex, cull=(i >= 20)) for i, mvp in enumerate(f["mvps"])]
+        spec.append((fb, rf.Context(clear=True), draws))
+    b = RB.Batch(spec)
+    b.run(sync=True)
+    stats = b.wait()
+    for k, f in enumerate(frames):
+        col = np.zeros((f["h"], f["w"], 4), np.uint8)
+        col[...] = (0, 0, 0, 255)
+        dep = np.zeros((f["h"], f["w"]), np.float32)
+        st = np.zeros(8, np.uint64)
+        calls = objs_o = 0
+        for i, mvp in enumerate(f["mvps"]):
+            if i >= 20 and O.bbox_visibility(bb, mvp) == O.HIDDEN:
+                continue
+            err, _ = O.render(3, ctri, cpos, cuv, mvp, f["vp"], col, depth=dep, tex=tex_px, stats=st)
+            assert err == 0
+            calls += 1
+            objs_o += i >= 20
+        assert 10 < objs_o < 200, objs_o  # the camera sees part of the field
+        gcol, gdep = fbs[k].read()
+        assert np.array_equal(gcol, col), f"frame {k} colour"
+        assert np.array_equal(gdep.view(np.uint32), dep.view(np.uint32)), f"frame {k} depth"
+        s = stats[k]
+        assert [s.calls, s.prims_i, s.prims_o, s.verts_i, s.verts_o, s.frags_i, s.frags_o] == \
+            [calls] + [int(x) for x in st[1:7]]
+        assert (s.objs_i, s.objs_o) == (len(models) - 20, objs_o)
+    # the serving loop culls the same way
+    srv = RB.Server(budget_bytes=1 << 30, max_batch=4, n_streams=1)
+    try:
+        f = frames[0]
+        draws = [RB.mesh_draw(cube, shd, mvp, f["vp"], texture=tex, cull=(i >= 20)) for i, mvp in enumerate(f["mvps"])]
+        srv.submit(0, draws, f["w"], f["h"])
+        buf = np.empty(f["w"] * f["h"] * 4, np.uint8)
+        res = srv.poll(1, timeout_ms=30000, buffers=[buf])
+        assert len(res) == 1 and res[0].status == 0
+        assert np.array_equal(buf.reshape(f["h"], f["w"], 4), fbs[0].read()[0])
+        assert (res[0].stats.calls, res[0].stats.objs_i, res[0].stats.objs_o, res[0].stats.frags_o) == \
+            (stats[0].calls, stats[0].objs_i, stats[0].objs_o, stats[0].frags_o)
+    finally:
+        srv.close()
+    b.free()
+    for fb in fbs:
+        fb.free()
+    cube.free()
+    tex.free()

@@ -10,6 +10,7 @@ import pytest
 
 import oracle_lib as O
 from retrofire_amd import mat as M
+from retrofire_amd import scenes as S
 
 F = np.float32
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -404,6 +405,84 @@ def test_rgb565_and_4444_drop_low_bits():
         if v % 17 == 0:
             p4 = int(_flat_fill(6, (v, 255 - v, v, 255 - v), (), np.uint16)[0, 0])
             assert p4 == ((v >> 4) << 12) | (((255 - v) >> 4) << 8) | ((v >> 4) << 4) | ((255 - v) >> 4)
+
+
+# ------------------------------------------------ scene.rs (object culling)
+
+INF = np.float32(np.inf)
+
+
+def test_bbox_of_extends_like_bbox_extend():  # scene.rs:144-153
+    b = O.bbox_of([(-1, -2, -3), (5, 3, 2), (1, 1, 1)])
+    assert b.tolist() == [-1, -2, -3, 5, 3, 2]
+    b = O.bbox_of([(-1, -2, -3), (5, 3, 2), (1, 1, 1), (-2, 3, 3)])
+    assert b.tolist() == [-2, -2, -3, 5, 3, 3]
+
+
+def test_bbox_of_empty_is_default_and_skips_nan():  # scene.rs:107-112, f32::min / f32::max
+    assert O.bbox_of(np.zeros((0, 3), np.float32)).tolist() == [INF, INF, INF, -INF, -INF, -INF]
+    b = O.bbox_of([(np.nan, 0, 0), (1, 2, np.nan), (3, np.nan, 5)])
+    assert b.tolist() == [1, 0, 0, 3, 2, 5]
+
+
+def test_bbox_visibility_closed_forms():
+    # Identity transform, w = 1: the frustum is the cube [-1, 1]^3, and a
+    # corner on a face has signed distance 0, which is not outside (clip.rs:109-111).
+    I = M.identity()
+    assert O.bbox_visibility([-1, -1, -1, 1, 1, 1], I) == O.VISIBLE
+    assert O.bbox_visibility([-.5, -.5, -.5, .5, .5, .5], I) == O.VISIBLE
+    assert O.bbox_visibility([1.5, -1, -1, 3, 1, 1], I) == O.HIDDEN
+    assert O.bbox_visibility([-1, -3, -1, 1, -1.01, 1], I) == O.HIDDEN
+    assert O.bbox_visibility([-1, -1, 1.25, 1, 1, 2], I) == O.HIDDEN
+    assert O.bbox_visibility([0, 0, 0, 2, 2, 2], I) == O.CLIPPED
+    # A box around the whole frustum: every corner is outside some plane but
+    # no plane has all corners outside it, so it is not Hidden (clip.rs:252-257).
+    assert O.bbox_visibility([-3, -3, -3, 3, 3, 3], I) == O.CLIPPED
+    # Each corner goes through the full projective transform (w from row 3):
+    # uniform scale of x, y, z and w by 4 changes nothing; a translation in
+    # the last column moves the box.
+    assert O.bbox_visibility([-1, -1, -1, 1, 1, 1], M.then(I, np.diag([4, 4, 4, 4]).astype(np.float32))) == O.VISIBLE
+    assert O.bbox_visibility([-1, -1, -1, 1, 1, 1], M.translate3(3, 0, 0)) == O.HIDDEN
+    assert O.bbox_visibility([-1, -1, -1, 1, 1, 1], M.translate3(1, 0, 0)) == O.CLIPPED
+
+
+def test_bbox_visibility_agrees_with_the_vertices():
+    """Brute force over random objects and cameras: Hidden implies every
+    vertex is outside one same plane and render() draws nothing; Visible
+    implies every vertex is inside; a vertex inside means not Hidden."""
+    import ctypes as C
+    rng = np.random.default_rng(5)
+    cpos, cuv, ctri = S.cube(1.0)
+    proj = M.perspective(np.float32(1.2), np.float32(1.5), 0.1, 50.0)
+    counts = [0, 0, 0]
+    for k in range(300):
+        t = rng.uniform([-12, -6, -8], [12, 6, 30]).astype(np.float32)
+        model = M.then(M.rotate_y(float(rng.uniform(0, 1))), M.translate3(*t))
+        pos = S.transform_points(M.then(np.diag([*rng.uniform(0.2, 3.0, 3), 1]).astype(np.float32), model), cpos)
+        mvp = M.then(M.rotate_y(float(rng.uniform(-0.1, 0.1))), proj)
+        st = O.bbox_visibility(O.bbox_of(pos), mvp)
+        counts[st] += 1
+        codes = []
+        for p in pos:
+            c = np.zeros(4, np.float32)
+            for i in range(4):
+                c[i] = np.float32(0)
+                for j in range(3):
+                    c[i] = np.float32(c[i] + np.float32(mvp.reshape(4, 4)[i, j] * p[j]))
+                c[i] = np.float32(c[i] + mvp.reshape(4, 4)[i, 3])
+            codes.append(O.lib().oracle_outcode(O._p(c, C.c_float)))
+        all_out = np.bitwise_and.reduce(codes)
+        if st == O.HIDDEN:
+            assert all_out != 0, k
+            fb = np.zeros((60, 90, 4), np.uint8)
+            err, s = O.render(3, ctri, pos, cuv, mvp, M.viewport(0, 60, 90, 0), fb, tex=np.zeros((2, 2, 4), np.uint8),
+                              ctx=(0, 0, 1, 1))
+            assert err == 0 and int(s[2]) == 0 and int(s[6]) == 0
+        if st == O.VISIBLE:
+            assert max(codes) == 0, k
+        if min(codes) == 0:
+            assert st != O.HIDDEN, k
+    assert min(counts) > 20, counts
 
 
 # ------------------------------------------- closed-form and brute-force pins
