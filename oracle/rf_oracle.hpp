@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 namespace oracle {
@@ -234,6 +235,53 @@ template <int N>
 struct TriC {
     ClipVert<N> v[3];
 };
+
+// Edge<ClipVert<A>> (geom Edge, prim.rs:41-60 Render::Clip).
+template <int N>
+struct EdgeC {
+    ClipVert<N> v[2];
+};
+
+// `Clip for [Edge<ClipVert<A>>]` (clip.rs:311-348): an edge with both ends
+// outside one plane is dropped, one with neither end outside any plane is
+// kept; otherwise it is cut plane by plane in PLANES order, the end outside
+// each plane being replaced by the intersection, and dropped if both ends
+// are outside some plane on the way (the outcodes consulted are those the
+// ends carry, recomputed for each intersection vertex).
+// `planes` lists PLANES indices (all six, in order, by default).
+template <int N>
+inline void clip_edges(const std::vector<EdgeC<N>>& edges, std::vector<EdgeC<N>>& out,
+                       const int* planes = nullptr, int np = 6) {
+    static const int ALL[6] = {0, 1, 2, 3, 4, 5};
+    if (!planes) planes = ALL;
+    for (const EdgeC<N>& e : edges) {
+        const bool both_outside = (e.v[0].outcode & e.v[1].outcode) != 0;
+        const bool neither_outside = (e.v[0].outcode | e.v[1].outcode) == 0;
+        ClipVert<N> a = e.v[0], b = e.v[1];
+        if (both_outside) continue;
+        if (neither_outside) {
+            out.push_back(EdgeC<N>{{a, b}});
+            continue;
+        }
+        bool dropped = false;
+        for (int i = 0; i < np; ++i) {
+            const int p = planes[i];
+            const uint8_t bit = (uint8_t)(1u << p);
+            const bool a_in = (bit & a.outcode) == 0;
+            const bool b_in = (bit & b.outcode) == 0;
+            if (!a_in && !b_in) {
+                dropped = true;
+                break;
+            }
+            ClipVert<N> v;
+            if (intersect<N>(p, a, b, v)) {
+                if (a_in) b = v;
+                else if (b_in) a = v;
+            }
+        }
+        if (!dropped) out.push_back(EdgeC<N>{{a, b}});
+    }
+}
 
 // `Clip for [Tri<ClipVert<A>>]` (clip.rs:350-400)
 template <int N>
@@ -557,6 +605,54 @@ inline void store_pixel(const Target& t, uint64_t x, uint64_t y, Rgba c) {
     }
 }
 
+// raster::line (raster.rs:122-177): a one-pixel-thick line. The ends are
+// ordered top to bottom; a line wider than tall is then drawn left to right,
+// one pixel per column x in (round_up_to_half(x0) as usize)..(round_up_to_half(x1)
+// as usize) on row `y as usize`, y stepped serially by dy/dx from the value
+// adjusted to the rounded x0 (dx and dy are taken before the left-right swap);
+// otherwise one pixel per row, x stepped serially by dx/dy. Every pixel's
+// scanline is `x..x + 1` with the single varying value (v0.pos, v0.attrib) of
+// the first end (`vary_to(vs, 1)` emits only its start), so a line has one
+// depth and one colour.
+template <int N, class F>
+inline void line(ScreenVert<N> v0, ScreenVert<N> v1, F&& scan_fn) {
+    if (v0.pos[1] > v1.pos[1]) std::swap(v0, v1);
+    const float dx = v1.pos[0] - v0.pos[0];
+    const float dy = v1.pos[1] - v0.pos[1];
+    auto scan = [&](uint64_t y, uint64_t x, const ScreenVert<N>& v) {
+        Scanline<N> sl;
+        sl.y = y;
+        sl.x0 = x;
+        sl.x1 = x + 1;
+        sl.v0 = Vy<N>{v.pos[0], v.pos[1], v.pos[2], v.attrib};
+        sl.dv_dx = vy_dvdt<N>(sl.v0, sl.v0, 1.0f);
+        sl.count = 1;
+        scan_fn(sl);
+    };
+    if (std::fabs(dx) > dy) {
+        if (dx < 0.0f) std::swap(v0, v1);
+        const float x0 = round_up_to_half(v0.pos[0]);
+        const float x1 = round_up_to_half(v1.pos[0]);
+        const float dy_dx = dy / dx;
+        const float y0 = v0.pos[1] + dy_dx * (x0 - v0.pos[0]);
+        float y = y0;
+        for (uint64_t x = as_usize(x0), xe = as_usize(x1); x < xe; ++x) {
+            scan(as_usize(y), x, v0);
+            y = y + dy_dx;
+        }
+    } else {
+        const float y0 = round_up_to_half(v0.pos[1]);
+        const float y1 = round_up_to_half(v1.pos[1]);
+        const float dx_dy = dx / dy;
+        const float x0 = v0.pos[0] + dx_dy * (y0 - v0.pos[1]);
+        float x = x0;
+        for (uint64_t y = as_usize(y0), ye = as_usize(y1); y < ye; ++y) {
+            scan(y, as_usize(x), v0);
+            x = x + dx_dy;
+        }
+    }
+}
+
 enum Err { OK = 0, E_INDEX = -1, E_BOUNDS = -2, E_ARG = -3 };
 
 // Framebuf::rasterize -> rasterize_fb (target.rs:163-198), or the colour-only
@@ -605,7 +701,9 @@ inline int rasterize_span(const Target& t, const Scanline<N>& sl, const Fs& fs, 
 // shader `vertex(mvp.apply(&v.pos), v.attrib)` (the form used by every
 // built-in shader; ProjMat3::apply mat.rs:965-969).
 //   tris:  n_tris index triples, `pos`: n_verts xyz, `attr`: n_verts x N.
-template <int N, bool PERSP, class Fs>
+//   P = 3: `tris` holds n_tris index triples (Tri<usize>); P = 2: index pairs
+//   (Edge<usize>, prim.rs:41-60: edge clipping, no backfaces, raster::line).
+template <int N, bool PERSP, class Fs, int P = 3>
 inline int render(const uint32_t* tris, uint64_t n_tris, const float* pos, const float* attr,
                   uint64_t n_verts, const float* mvp, const float* to_screen_m, const Fs& fs,
                   const Target& t, const Ctx& ctx, Stats& out_stats) {
@@ -622,6 +720,40 @@ inline int render(const uint32_t* tris, uint64_t n_tris, const float* pos, const
         Attr<N> a;
         for (int k = 0; k < N; ++k) a.c[k] = attr[N * i + k];
         verts[i] = clip_vert<N>(clip, a);
+    }
+    if (P == 2) {
+        std::vector<EdgeC<N>> edges(n_tris);
+        for (uint64_t i = 0; i < n_tris; ++i)
+            for (int k = 0; k < 2; ++k) {
+                const uint32_t ix = tris[2 * i + k];
+                if (ix >= n_verts) return E_INDEX;
+                edges[i].v[k] = verts[ix];
+            }
+        std::vector<EdgeC<N>> clipped;
+        clipped.reserve(n_tris / 2);
+        clip_edges<N>(edges, clipped);
+        int err = OK;
+        for (const EdgeC<N>& c : clipped) {
+            const ScreenVert<N> s0 = to_screen<N, PERSP>(c.v[0], to_screen_m);
+            const ScreenVert<N> s1 = to_screen<N, PERSP>(c.v[1], to_screen_m);
+            // Render::is_backface defaults to false for edges (render.rs:72-75)
+            if (face_culled(ctx.face_cull, false)) continue;
+            st.prims_o += 1;
+            st.verts_o += 3;  // render.rs:193 counts 3 for every primitive
+            line<N>(s0, s1, [&](const Scanline<N>& sl) {
+                if (err != OK) return;
+                err = rasterize_span<N, PERSP>(t, sl, fs, ctx, st);
+            });
+            if (err != OK) return err;
+        }
+        out_stats.calls += st.calls;
+        out_stats.prims_i += st.prims_i;
+        out_stats.prims_o += st.prims_o;
+        out_stats.verts_i += st.verts_i;
+        out_stats.verts_o += st.verts_o;
+        out_stats.frags_i += st.frags_i;
+        out_stats.frags_o += st.frags_o;
+        return OK;
     }
     // 2. primitive assembly (render.rs:167-172); indexing panics when out of range
     std::vector<TriC<N>> prims(n_tris);

@@ -24,12 +24,14 @@ struct OracleCtx {
 // One render() call. Stats are accumulated into stats[0..7]:
 // calls, prims_i, prims_o, verts_i, verts_o, frags_i, frags_o.
 // Returns 0, or -1 (index out of range), -2 (span outside target), -3 (bad arg).
-int oracle_render(int kind, const uint32_t* tris, uint64_t n_tris, const float* pos,
-                  const float* attr, uint64_t n_verts, const float* mvp, const float* viewport,
-                  const uint8_t* flat_rgba, const uint8_t* tex, uint32_t tex_w, uint32_t tex_h,
-                  uint32_t tex_stride, uint8_t* color, uint64_t width, uint64_t height,
-                  uint64_t color_stride_bytes, int32_t fmt, float* depth,
-                  uint64_t depth_stride, const OracleCtx* c, uint64_t* stats) {
+// `prim` 0: `tris` holds index triples (Tri<usize>); 1: index pairs (Edge<usize>).
+int oracle_render_prims(int prim, int kind, const uint32_t* tris, uint64_t n_tris, const float* pos,
+                        const float* attr, uint64_t n_verts, const float* mvp, const float* viewport,
+                        const uint8_t* flat_rgba, const uint8_t* tex, uint32_t tex_w, uint32_t tex_h,
+                        uint32_t tex_stride, uint8_t* color, uint64_t width, uint64_t height,
+                        uint64_t color_stride_bytes, int32_t fmt, float* depth,
+                        uint64_t depth_stride, const OracleCtx* c, uint64_t* stats) {
+    if (prim != 0 && prim != 1) return E_ARG;
     Target t;
     t.color = color;
     t.width = width;
@@ -48,17 +50,20 @@ int oracle_render(int kind, const uint32_t* tris, uint64_t n_tris, const float* 
     switch (kind) {
         case K_FLAT: {
             FsFlat fs{Rgba{flat_rgba[0], flat_rgba[1], flat_rgba[2], flat_rgba[3]}};
-            err = render<0, false>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st);
+            err = prim ? render<0, false, FsFlat, 2>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st)
+                       : render<0, false>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st);
             break;
         }
         case K_GOURAUD: {
             FsGouraud fs;
-            err = render<3, false>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st);
+            err = prim ? render<3, false, FsGouraud, 2>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st)
+                       : render<3, false>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st);
             break;
         }
         case K_TEX_CLAMP: {
             FsTexClamp fs{Texture{tex, tex_w, tex_h, tex_stride}};
-            err = render<2, true>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st);
+            err = prim ? render<2, true, FsTexClamp, 2>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st)
+                       : render<2, true>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st);
             break;
         }
         case K_TEX_REPEAT_POT: {
@@ -66,7 +71,8 @@ int oracle_render(int kind, const uint32_t* tris, uint64_t n_tris, const float* 
             if (tex_w == 0 || tex_h == 0 || (tex_w & (tex_w - 1)) || (tex_h & (tex_h - 1)))
                 return E_ARG;
             FsTexRepeatPot fs{Texture{tex, tex_w, tex_h, tex_stride}};
-            err = render<2, true>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st);
+            err = prim ? render<2, true, FsTexRepeatPot, 2>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st)
+                       : render<2, true>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st);
             break;
         }
         default:
@@ -82,6 +88,18 @@ int oracle_render(int kind, const uint32_t* tris, uint64_t n_tris, const float* 
         stats[6] += st.frags_o;
     }
     return err;
+}
+
+// One render() call over triangles (oracle_render_prims with prim 0).
+int oracle_render(int kind, const uint32_t* tris, uint64_t n_tris, const float* pos,
+                  const float* attr, uint64_t n_verts, const float* mvp, const float* viewport,
+                  const uint8_t* flat_rgba, const uint8_t* tex, uint32_t tex_w, uint32_t tex_h,
+                  uint32_t tex_stride, uint8_t* color, uint64_t width, uint64_t height,
+                  uint64_t color_stride_bytes, int32_t fmt, float* depth,
+                  uint64_t depth_stride, const OracleCtx* c, uint64_t* stats) {
+    return oracle_render_prims(0, kind, tris, n_tris, pos, attr, n_verts, mvp, viewport, flat_rgba, tex, tex_w,
+                               tex_h, tex_stride, color, width, height, color_stride_bytes, fmt, depth,
+                               depth_stride, c, stats);
 }
 
 // The `fp` variant of core/examples/hello_tri.rs:11-19: vertex colours go
@@ -227,6 +245,56 @@ int64_t oracle_clip_tris_f32(const float* in, int64_t n_in, const int32_t* plane
             p[4] = res[i].v[k].attrib.c[0];
         }
     return (int64_t)res.size();
+}
+
+// `[edges].clip(planes, out)` for ClipVert<f32> (clip.rs:311-348). `in` holds
+// n_in edges of 2 x (x, y, z, w, attr); returns the output count (or -1).
+int64_t oracle_clip_edges_f32(const float* in, int64_t n_in, const int32_t* planes, int32_t np,
+                              float* out, int64_t max_out) {
+    std::vector<EdgeC<1>> edges(n_in), res;
+    for (int64_t i = 0; i < n_in; ++i)
+        for (int k = 0; k < 2; ++k) {
+            const float* p = in + (i * 2 + k) * 5;
+            Attr<1> a{{p[4]}};
+            edges[i].v[k] = clip_vert<1>(p, a);
+        }
+    std::vector<int> pl(planes, planes + np);
+    clip_edges<1>(edges, res, pl.data(), np);
+    if ((int64_t)res.size() > max_out) return -1;
+    for (size_t i = 0; i < res.size(); ++i)
+        for (int k = 0; k < 2; ++k) {
+            float* p = out + (i * 2 + k) * 5;
+            for (int c = 0; c < 4; ++c) p[c] = res[i].v[k].pos[c];
+            p[4] = res[i].v[k].attrib.c[0];
+        }
+    return (int64_t)res.size();
+}
+
+// raster::line (raster.rs:122-177) between screen vertices a and b, each
+// (x, y, z, var). Writes up to max_out pixels as (x, y) and each fragment's
+// (z, var); returns the pixel count (or -1 if more).
+int64_t oracle_line_f32(const float* a, const float* b, int64_t* out_xy, float* out_zv, int64_t max_out) {
+    ScreenVert<1> v0, v1;
+    for (int c = 0; c < 3; ++c) {
+        v0.pos[c] = a[c];
+        v1.pos[c] = b[c];
+    }
+    v0.attrib.c[0] = a[3];
+    v1.attrib.c[0] = b[3];
+    int64_t n = 0;
+    bool over = false;
+    line<1>(v0, v1, [&](const Scanline<1>& sl) {
+        if (n >= max_out) {
+            over = true;
+            return;
+        }
+        out_xy[2 * n] = (int64_t)sl.x0;
+        out_xy[2 * n + 1] = (int64_t)sl.y;
+        out_zv[2 * n] = sl.v0.z;
+        out_zv[2 * n + 1] = sl.v0.v.c[0];
+        ++n;
+    });
+    return over ? -1 : n;
 }
 
 // ClipPlane::clip_simple_polygon for one plane over an n-vertex polygon

@@ -40,6 +40,12 @@ def lib():
                                     u8p, u8p, C.c_uint32, C.c_uint32, C.c_uint32, u8p, C.c_uint64,
                                     C.c_uint64, C.c_uint64, C.c_int32, f32p, C.c_uint64,
                                     C.POINTER(OracleCtx), u64p]
+        L.oracle_clip_edges_f32.restype = C.c_int64
+        L.oracle_clip_edges_f32.argtypes = [f32p, C.c_int64, C.POINTER(C.c_int32), C.c_int32, f32p, C.c_int64]
+        L.oracle_line_f32.restype = C.c_int64
+        L.oracle_line_f32.argtypes = [f32p, f32p, i64p, f32p, C.c_int64]
+        L.oracle_render_prims.restype = C.c_int
+        L.oracle_render_prims.argtypes = [C.c_int] + L.oracle_render.argtypes
         L.oracle_render_hello_fp.restype = C.c_int
         L.oracle_render_hello_fp.argtypes = [u32p, C.c_uint64, f32p, f32p, C.c_uint64, f32p, f32p,
                                              u8p, C.c_uint64, C.c_uint64, u64p]
@@ -80,9 +86,10 @@ FMT_BPP = {0: 4, 1: 3, 2: 4, 3: 4, 4: 4, 5: 4, 6: 2, 7: 2}
 
 
 def render(kind, tris, pos, attr, mvp, vp, color, depth=None, fmt=0, flat=(255, 255, 255, 255),
-           tex=None, ctx=(1, 1, 1, 1), stats=None):
-    """One render() call on the oracle. `color` is an (h, w, bpp) uint8 or (h, w) uint32
-    array updated in place; `depth` an (h, w) float32 array or None."""
+           tex=None, ctx=(1, 1, 1, 1), stats=None, prim=0):
+    """One render() call on the oracle. `color` is an (h, w, bpp) uint8, (h, w)
+    uint32 or (h, w) uint16 array updated in place; `depth` an (h, w) float32
+    array or None. `prim` 0: `tris` holds index triples; 1: index pairs (edges)."""
     L = lib()
     tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1)
     pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
@@ -101,7 +108,7 @@ def render(kind, tris, pos, attr, mvp, vp, color, depth=None, fmt=0, flat=(255, 
     c = OracleCtx(*ctx)
     assert color.flags["C_CONTIGUOUS"]
     stride_bytes = color.strides[0]
-    err = L.oracle_render(kind, _p(tris, C.c_uint32), tris.size // 3, _p(pos, C.c_float),
+    err = L.oracle_render_prims(prim, kind, _p(tris, C.c_uint32), tris.size // (2 if prim else 3), _p(pos, C.c_float),
                           _p(attr, C.c_float), pos.size // 3, _p(mvp, C.c_float), _p(vp, C.c_float),
                           _p(flat, C.c_uint8), _p(texa, C.c_uint8), tw, th, tw,
                           color.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, stride_bytes, fmt,

@@ -81,6 +81,154 @@ def test_edge_clip():  # clip.rs:455-486
     np.testing.assert_array_equal(r[:2], np.array([vtx(-0.25, 0.75, 1.0), vtx(-1, 1, 0)], np.float32))
 
 
+def clip_edges(edges, planes):
+    import ctypes as C
+    L = O.lib()
+    arr = np.array(edges, dtype=np.float32).reshape(-1)
+    pl = np.array(planes, dtype=np.int32)
+    out = np.zeros(64 * 10, dtype=np.float32)
+    n = L.oracle_clip_edges_f32(O._p(arr, C.c_float), len(edges), pl.ctypes.data_as(C.POINTER(C.c_int32)),
+                                len(planes), O._p(out, C.c_float), 64)
+    assert n >= 0
+    return out[: n * 10].reshape(n, 2, 5)
+
+
+def test_edge_list_clip_far_plane():
+    # The four edges of clip.rs:455-486 through `[Edge].clip` instead of the
+    # polygon clipper: the intersections are the same ClipPlane::intersect
+    # values; an edge stays an edge (no extra vertex), and the inside end is kept.
+    e = [vtx(2, 0, -1), vtx(-1, 1, 1)]
+    np.testing.assert_array_equal(clip_edges([e], [FAR]), np.array([e], np.float32))
+    assert clip_edges([[vtx(2, 0, 1.5), vtx(-1, 1, 2)]], [FAR]).shape[0] == 0
+    r = clip_edges([[vtx(2, 0, 0), vtx(-1, 1, 2)]], [FAR])
+    np.testing.assert_array_equal(r, np.array([[vtx(2, 0, 0), vtx(0.5, 0.5, 1.0)]], np.float32))
+    r = clip_edges([[vtx(2, 0, 4), vtx(-1, 1, 0)]], [FAR])
+    np.testing.assert_array_equal(r, np.array([[vtx(-0.25, 0.75, 1.0), vtx(-1, 1, 0)]], np.float32))
+
+
+def test_edge_list_clip_frustum_brute_force():
+    """Random edges against all six planes: a kept edge lies on its input
+    segment and inside every plane (to rounding); an end inside the frustum is
+    kept bit for bit; an edge is dropped only if no point of it is inside
+    (checked on 2001 samples with a margin)."""
+    rng = np.random.default_rng(11)
+    ALL = list(range(6))
+    kept = dropped = 0
+    for _ in range(400):
+        a = rng.uniform(-2.5, 2.5, 3)
+        b = rng.uniform(-2.5, 2.5, 3)
+        e = [vtx(*a, a[0]), vtx(*b, b[0])]
+        r = clip_edges([e], ALL)
+        pa, pb = np.array(e[0][:4], np.float64), np.array(e[1][:4], np.float64)
+        t = np.linspace(0, 1, 2001)[:, None]
+        pts = pa + (pb - pa) * t
+        inside = np.all(np.abs(pts[:, :3]) <= pts[:, 3:4] - 1e-4, axis=1)
+        if r.shape[0] == 0:
+            dropped += 1
+            assert not inside.any(), (a, b)
+            continue
+        kept += 1
+        d = pb - pa
+        for q in r[0]:
+            qp = q[:4].astype(np.float64)
+            assert np.all(np.abs(qp[:3]) <= qp[3] + 1e-5)
+            s_ = np.dot(qp - pa, d) / np.dot(d, d)
+            assert -1e-5 <= s_ <= 1 + 1e-5 and np.allclose(pa + s_ * d, qp, atol=1e-5)
+            assert abs(q[4] - q[0]) <= 1e-5  # the varying (x at the ends) is interpolated with the position
+        for k, end in enumerate((a, b)):
+            if np.all(np.abs(end) <= 1.0):
+                np.testing.assert_array_equal(r[0][k], np.array(e[k], np.float32))
+    assert kept > 50 and dropped > 50, (kept, dropped)
+
+
+def line_pixels(a, b):
+    import ctypes as C
+    aa = np.array(a, np.float32)
+    bb = np.array(b, np.float32)
+    xy = np.zeros(2 * 4096, np.int64)
+    zv = np.zeros(2 * 4096, np.float32)
+    n = O.lib().oracle_line_f32(O._p(aa, C.c_float), O._p(bb, C.c_float), xy.ctypes.data_as(C.POINTER(C.c_int64)),
+                                O._p(zv, C.c_float), 4096)
+    assert n >= 0
+    return xy[: 2 * n].reshape(n, 2), zv[: 2 * n].reshape(n, 2)
+
+
+def test_line_axis_aligned_closed_forms():
+    # Wide: columns (x0 + 0.5).floor() .. (x1 + 0.5).floor(), row floor(y);
+    # tall: rows likewise, column floor(x); either end order gives the same pixels.
+    for a, b in (((2.3, 5.7), (9.6, 5.7)), ((9.6, 5.7), (2.3, 5.7))):
+        xy, _ = line_pixels((*a, 0.5, 0), (*b, 0.5, 0))
+        assert xy.tolist() == [[x, 5] for x in range(2, 10)]
+    for a, b in (((4.2, 1.2), (4.2, 6.9)), ((4.2, 6.9), (4.2, 1.2))):
+        xy, _ = line_pixels((*a, 0.5, 0), (*b, 0.5, 0))
+        assert xy.tolist() == [[4, y] for y in range(1, 7)]
+    # a pixel centre exactly on the start is inside, on the end outside
+    xy, _ = line_pixels((2.5, 3.0, 1, 0), (6.5, 3.0, 1, 0))
+    assert xy[:, 0].tolist() == [3, 4, 5, 6]
+    # degenerate: one point draws nothing
+    assert line_pixels((3.3, 3.3, 1, 0), (3.3, 3.3, 1, 0))[0].shape[0] == 0
+
+
+def test_line_brute_force_geometry():
+    """Random lines: one pixel per column (wide) or row (tall) over the
+    rounded range, each within half a pixel (plus rounding drift) of the
+    exact line at the pixel's centre column or row."""
+    rng = np.random.default_rng(12)
+    for _ in range(500):
+        a = rng.uniform(0, 200, 2)
+        b = rng.uniform(0, 200, 2)
+        xy, _ = line_pixels((*a, 1, 0), (*b, 1, 0))
+        (ax, ay), (bx, by) = a.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64)
+        if abs(bx - ax) > abs(by - ay):
+            lo, hi = sorted((ax, bx))
+            cols = np.arange(int(np.floor(lo + 0.5)), int(np.floor(hi + 0.5)))
+            assert xy[:, 0].tolist() == cols.tolist()
+            yl = ay + (by - ay) * ((cols + 0.5) - ax) / (bx - ax)
+            assert np.all(np.abs(yl - (xy[:, 1] + 0.5)) <= 0.5 + 1e-3)
+        else:
+            lo, hi = sorted((ay, by))
+            rows = np.arange(int(np.floor(lo + 0.5)), int(np.floor(hi + 0.5)))
+            assert xy[:, 1].tolist() == rows.tolist()
+            xl = ax + (bx - ax) * ((rows + 0.5) - ay) / (by - ay)
+            assert np.all(np.abs(xl - (xy[:, 0] + 0.5)) <= 0.5 + 1e-3)
+
+
+def test_line_takes_depth_and_varying_of_its_first_end():
+    # Every fragment of a line carries (v0.pos, v0.attrib), v0 being the left
+    # end of a wide line and the top end of a tall one (raster.rs:126-162).
+    _, zv = line_pixels((10.2, 20.0, 0.25, 7.0), (50.7, 30.0, 0.5, 9.0))  # wide, left end first
+    assert (zv == (0.25, 7.0)).all()
+    _, zv = line_pixels((50.7, 20.0, 0.25, 7.0), (10.2, 30.0, 0.5, 9.0))  # wide, left end is b
+    assert (zv == (0.5, 9.0)).all()
+    _, zv = line_pixels((10.0, 40.2, 0.25, 7.0), (15.0, 2.7, 0.5, 9.0))  # tall, top end is b
+    assert (zv == (0.5, 9.0)).all() and zv.shape[0] == 37  # rows 3..39
+
+
+def test_edges_render_stats_and_culling():
+    # Edges through render(): prims.o counts clipped edges (verts.o 3 each,
+    # render.rs:193), no edge is a backface, so FaceCull::Front drops all of
+    # them (ctx.rs:95-101); colour comes from the first end.
+    w, h = 64, 48
+    vp = M.viewport(0, h, w, 0)
+    pos = np.array([(-0.5, -0.5, 0), (0.5, 0.25, 0), (0.0, 0.9, 0), (3, 3, 0), (4, 3, 0)], np.float32)
+    col = np.array([(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1), (1, 1, 1)], np.float32) * 0.99
+    edges = np.array([[0, 1], [1, 2], [2, 0], [3, 4]], np.uint32)
+    fb = np.zeros((h, w, 4), np.uint8)
+    dep = np.zeros((h, w), np.float32)
+    err, st = O.render(1, edges, pos, col, M.identity(), vp, fb, depth=dep, ctx=(0, 0, 1, 1), prim=1)
+    assert err == 0
+    assert st[:5].tolist() == [1, 4, 3, 5, 9]
+    # no depth test: every fragment is written; the triangle's corners are
+    # shared pixels of two edges
+    lit = int((fb[..., 3] == 255).sum())
+    assert int(st[5]) == int(st[6]) and lit <= int(st[6]) <= lit + 3 and lit > 60
+    fb2 = np.zeros((h, w, 4), np.uint8)
+    err, st2 = O.render(1, edges, pos, col, M.identity(), vp, fb2, ctx=(0, 2, 1, 1), prim=1)
+    assert err == 0 and int(st2[2]) == 0 and not fb2.any()
+    err, _ = O.render(1, np.array([[0, 9]], np.uint32), pos, col, M.identity(), vp, fb2, prim=1)
+    assert err == -1
+
+
 def _tri(a, b, c):
     return [vtx(*a), vtx(*b), vtx(*c)]
 
