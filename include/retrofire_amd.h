@@ -116,10 +116,18 @@ typedef struct rf_target {
 /* One render() call with host-resident data: vertices are position (x,y,z)
  * plus an attribute of 0 (flat), 3 (Gouraud, Color3f) or 2 (textured,
  * TexCoord) floats. */
+/* Primitive types (the Render impls for Tri<usize> and Edge<usize>,
+ * prim.rs:11-60). */
+typedef enum rf_prim {
+    RF_PRIM_TRI = 0,    /* index triples; tri_fill (raster.rs:185-302)          */
+    RF_PRIM_EDGE = 1    /* index pairs; edge clipping (clip.rs:311-348), no
+                         * backfaces, raster::line (raster.rs:122-177)          */
+} rf_prim;
+
 typedef struct rf_draw {
     uint32_t kind;                  /* rf_shader_kind */
-    const uint32_t* tris;           /* n_tris x 3 indices */
-    uint64_t n_tris;
+    const uint32_t* tris;           /* n_tris x 3 indices (x 2 for RF_PRIM_EDGE) */
+    uint64_t n_tris;                /* primitives */
     const float* pos;               /* n_verts x 3 */
     const float* attr;              /* n_verts x attribute floats */
     uint64_t n_verts;
@@ -128,6 +136,7 @@ typedef struct rf_draw {
     uint8_t flat_color[4];          /* RF_SHADER_FLAT: Color4 r, g, b, a */
     const uint8_t* tex;             /* textured: Color4 texels, row-major */
     uint32_t tex_w, tex_h, tex_stride; /* stride in texels */
+    uint32_t prim;                  /* rf_prim */
 } rf_draw;
 
 typedef struct rf_device rf_device;
@@ -164,6 +173,10 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
  * n_attr is the attribute float count of the kinds the mesh is drawn with. */
 rf_status rf_mesh_create(rf_device* dev, const float* pos, const float* attr, uint32_t n_attr,
                          uint64_t n_verts, const uint32_t* tris, uint64_t n_tris, rf_mesh** out);
+/* A resident mesh of edges (Edge<usize> index pairs, as debug.rs batches
+ * hold): draws over it render lines. */
+rf_status rf_mesh_create_edges(rf_device* dev, const float* pos, const float* attr, uint32_t n_attr,
+                               uint64_t n_verts, const uint32_t* edges, uint64_t n_edges, rf_mesh** out);
 rf_status rf_mesh_free(rf_device* dev, rf_mesh* mesh);
 rf_status rf_texture_create(rf_device* dev, const uint8_t* rgba, uint32_t w, uint32_t h,
                             uint32_t stride_texels, rf_texture** out);
@@ -186,7 +199,7 @@ typedef struct rf_mesh_draw {
     float viewport[16];
     uint8_t flat_color[4];
     const rf_texture* tex;
-    uint64_t first_tri;      /* draw triangles [first_tri, first_tri + n_tris) */
+    uint64_t first_tri;      /* draw primitives [first_tri, first_tri + n_tris) of the mesh */
     uint64_t n_tris;         /* 0: to the end of the mesh */
     /* 1: whole-object culling on the device, as callers do before render()
      * (crates.rs:118-122): the draw is skipped, and counts in no stats but

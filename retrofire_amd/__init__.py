@@ -201,11 +201,14 @@ def make_target(target) -> _ffi.RfTarget:
 
 
 def make_draw(prims, verts, shd, uniform, to_screen, keep: list) -> _ffi.RfDraw:
-    tris = np.ascontiguousarray(prims, dtype=np.uint32).reshape(-1, 3)
+    prims = np.asarray(prims)
+    edges = prims.ndim == 2 and prims.shape[1] == 2
+    tris = np.ascontiguousarray(prims, dtype=np.uint32).reshape(-1, 2 if edges else 3)
     pos, attr = verts if isinstance(verts, tuple) else (verts, None)
     pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
     d = _ffi.RfDraw()
     d.kind = shd.kind
+    d.prim = 1 if edges else 0
     d.tris = tris.ctypes.data_as(C.POINTER(C.c_uint32))
     d.n_tris = tris.shape[0]
     d.pos = _fptr(pos)
@@ -235,7 +238,8 @@ def make_draw(prims, verts, shd, uniform, to_screen, keep: list) -> _ffi.RfDraw:
 def render(prims, verts, shd, uniform, to_screen, target, ctx: Context, device: Device | None = None):
     """render(prims, verts, shader, uniform, to_screen, target, ctx)
     (core/src/render.rs:134-207) on the GPU. `prims` is an (n, 3) index
-    array; `verts` is positions (n, 3) or a tuple (positions, attributes);
+    array of triangles (Tri<usize>) or an (n, 2) one of edges (Edge<usize>,
+    drawn as lines); `verts` is positions (n, 3) or a tuple (positions, attributes);
     `uniform` is the mvp matrix of the built-in vertex shader; `to_screen` the
     viewport matrix. The target's arrays are updated in place; ctx.stats
     accumulates the call's statistics."""

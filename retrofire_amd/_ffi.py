@@ -37,7 +37,7 @@ class RfDraw(C.Structure):
                 ("pos", C.POINTER(C.c_float)), ("attr", C.POINTER(C.c_float)), ("n_verts", C.c_uint64),
                 ("mvp", C.c_float * 16), ("viewport", C.c_float * 16), ("flat_color", C.c_uint8 * 4),
                 ("tex", C.POINTER(C.c_uint8)), ("tex_w", C.c_uint32), ("tex_h", C.c_uint32),
-                ("tex_stride", C.c_uint32)]
+                ("tex_stride", C.c_uint32), ("prim", C.c_uint32)]
 
 
 class RfMeshDraw(C.Structure):
@@ -76,6 +76,8 @@ SIGNATURES = {
     "rf_render": (_st, [_vp, C.POINTER(RfDraw), C.POINTER(RfTarget), C.POINTER(RfCtx), C.POINTER(RfStats)]),
     "rf_mesh_create": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.c_uint64,
                              C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(_vp)]),
+    "rf_mesh_create_edges": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.c_uint64,
+                                   C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(_vp)]),
     "rf_mesh_free": (_st, [_vp, _vp]),
     "rf_texture_create": (_st, [_vp, C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.POINTER(_vp)]),

@@ -14,12 +14,15 @@ BPP = {0: 4, 1: 3, 2: 4, 3: 4, 4: 4, 5: 4, 6: 2, 7: 2}
 
 class Mesh:
     """A device-resident mesh (geom/mesh.rs:24-30): positions, an optional
-    per-vertex attribute (n_attr floats) and index triples."""
+    per-vertex attribute (n_attr floats) and index triples, or index pairs
+    (an (n, 2) array: edges, drawn as lines)."""
 
     def __init__(self, pos, tris, attr=None, device=None):
         self.dev = device or default_device()
         pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
-        tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+        tris = np.asarray(tris)
+        self.edges = tris.ndim == 2 and tris.shape[1] == 2
+        tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 2 if self.edges else 3)
         n_attr = 0
         ap = None
         if attr is not None:
@@ -28,9 +31,9 @@ class Mesh:
             ap = attr.ctypes.data_as(C.POINTER(C.c_float))
         self.n_verts, self.n_tris, self.n_attr = pos.shape[0], tris.shape[0], n_attr
         self.h = C.c_void_p()
-        check(_ffi.lib().rf_mesh_create(self.dev.h, pos.ctypes.data_as(C.POINTER(C.c_float)), ap, n_attr,
-                                        pos.shape[0], tris.ctypes.data_as(C.POINTER(C.c_uint32)), tris.shape[0],
-                                        C.byref(self.h)))
+        create = _ffi.lib().rf_mesh_create_edges if self.edges else _ffi.lib().rf_mesh_create
+        check(create(self.dev.h, pos.ctypes.data_as(C.POINTER(C.c_float)), ap, n_attr, pos.shape[0],
+                     tris.ctypes.data_as(C.POINTER(C.c_uint32)), tris.shape[0], C.byref(self.h)))
 
     def free(self):
         if self.h:

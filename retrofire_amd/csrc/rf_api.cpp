@@ -277,6 +277,8 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     rf_status st = check_ctx(ctx);
     if (st) return st;
     if (draw->kind > KIND_TEX_REPEAT_POT) return RF_E_UNSUPPORTED;
+    if (draw->prim > PRIM_EDGE) return RF_E_ARG;
+    const uint32_t ipp = draw->prim == PRIM_EDGE ? 2u : 3u;  // indices per primitive
     if ((st = check_tex(draw->kind, draw->tex, draw->tex_w, draw->tex_h, draw->tex_stride))) return st;
     if (!target->color || target->width == 0 || target->height == 0 || target->fmt > PIX_LAST ||
         target->width > MAX_DIM || target->height > MAX_DIM)
@@ -292,7 +294,7 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     hipStream_t s = d.stream;
 
     // uploads
-    const size_t pos_b = draw->n_verts * 12, attr_b = draw->n_verts * na * 4, tri_b = draw->n_tris * 12;
+    const size_t pos_b = draw->n_verts * 12, attr_b = draw->n_verts * na * 4, tri_b = draw->n_tris * ipp * 4;
     const bool textured = draw->kind >= KIND_TEX_CLAMP;
     const size_t tex_b = textured ? (size_t)draw->tex_h * draw->tex_stride * 4 : 0;
     const size_t col_b = (size_t)target->height * target->color_stride;
@@ -317,6 +319,7 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
 
     DrawDesc dd;
     std::memset(&dd, 0, sizeof(dd));
+    dd.prim = draw->prim;
     dd.pos = (const float*)d.up_pos.p;
     dd.attr = (const float*)d.up_attr.p;
     dd.tris = (const uint32_t*)d.up_tris.p;
@@ -397,8 +400,10 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
 
 // ---- residency -----------------------------------------------------------
 
-rf_status rf_mesh_create(rf_device* dev, const float* pos, const float* attr, uint32_t n_attr,
-                         uint64_t n_verts, const uint32_t* tris, uint64_t n_tris, rf_mesh** out) {
+static rf_status mesh_create(rf_device* dev, const float* pos, const float* attr, uint32_t n_attr,
+                             uint64_t n_verts, const uint32_t* tris, uint64_t n_tris, uint32_t prim,
+                             rf_mesh** out) {
+    const uint64_t ipp = prim == PRIM_EDGE ? 2 : 3;  // indices per primitive
     if (!dev || !out || (n_verts && !pos) || (n_tris && !tris) || (n_attr && n_verts && !attr) || n_attr > 3)
         return RF_E_ARG;
     if (n_verts > 0xFFFFFFFFull || n_tris > 0xFFFFFFFFull) return RF_E_ARG;
@@ -409,8 +414,9 @@ rf_status rf_mesh_create(rf_device* dev, const float* pos, const float* attr, ui
     m->n_verts = n_verts;
     m->n_tris = n_tris;
     m->n_attr = n_attr;
+    m->prim = prim;
     const size_t pb = std::max<size_t>(n_verts * 12, 4), ab = std::max<size_t>(n_verts * n_attr * 4, 4),
-                 tb = std::max<size_t>(n_tris * 12, 4);
+                 tb = std::max<size_t>(n_tris * ipp * 4, 4);
     if (m->pos.ensure(pb, &d.bytes) || m->attr.ensure(ab, &d.bytes) || m->tris.ensure(tb, &d.bytes)) {
         m->pos.release(&d.bytes);
         m->attr.release(&d.bytes);
@@ -421,7 +427,7 @@ rf_status rf_mesh_create(rf_device* dev, const float* pos, const float* attr, ui
     hipError_t e = hipSuccess;
     if (n_verts) e = hipMemcpy(m->pos.p, pos, n_verts * 12, hipMemcpyHostToDevice);
     if (!e && n_verts && n_attr) e = hipMemcpy(m->attr.p, attr, n_verts * n_attr * 4, hipMemcpyHostToDevice);
-    if (!e && n_tris) e = hipMemcpy(m->tris.p, tris, n_tris * 12, hipMemcpyHostToDevice);
+    if (!e && n_tris) e = hipMemcpy(m->tris.p, tris, n_tris * ipp * 4, hipMemcpyHostToDevice);
     if (e) {
         rf_mesh_free(dev, reinterpret_cast<rf_mesh*>(m));
         return RF_E_DEVICE;
@@ -439,6 +445,16 @@ rf_status rf_mesh_create(rf_device* dev, const float* pos, const float* attr, ui
         }
     *out = reinterpret_cast<rf_mesh*>(m);
     return RF_OK;
+}
+
+rf_status rf_mesh_create(rf_device* dev, const float* pos, const float* attr, uint32_t n_attr,
+                         uint64_t n_verts, const uint32_t* tris, uint64_t n_tris, rf_mesh** out) {
+    return mesh_create(dev, pos, attr, n_attr, n_verts, tris, n_tris, PRIM_TRI, out);
+}
+
+rf_status rf_mesh_create_edges(rf_device* dev, const float* pos, const float* attr, uint32_t n_attr,
+                               uint64_t n_verts, const uint32_t* edges, uint64_t n_edges, rf_mesh** out) {
+    return mesh_create(dev, pos, attr, n_attr, n_verts, edges, n_edges, PRIM_EDGE, out);
 }
 
 rf_status rf_mesh_free(rf_device* dev, rf_mesh* mesh) {

@@ -72,7 +72,8 @@ static rf_status build_descs(Batch& b, uint32_t tile_w) {
             const uint64_t nt = md.n_tris ? md.n_tris : m->n_tris - first;
             dd.pos = (const float*)m->pos.p;
             dd.attr = (const float*)m->attr.p;
-            dd.tris = (const uint32_t*)m->tris.p + 3 * first;
+            dd.tris = (const uint32_t*)m->tris.p + (m->prim == PRIM_EDGE ? 2 : 3) * first;
+            dd.prim = m->prim;
             dd.tex = t ? (const uint8_t*)t->data.p : nullptr;
             dd.n_verts = (uint32_t)m->n_verts;
             dd.n_tris = (uint32_t)nt;

@@ -50,8 +50,9 @@ struct Device {
 
 struct Mesh {
     DevBuf pos, attr, tris;
-    uint64_t n_verts = 0, n_tris = 0;
+    uint64_t n_verts = 0, n_tris = 0;  // n_tris: primitives (triangles or edges)
     uint32_t n_attr = 0;
+    uint32_t prim = 0;                  // PrimType
     float bbox[6] = {0, 0, 0, 0, 0, 0};  // BBox::of over the positions (scene.rs:37-39)
 };
 

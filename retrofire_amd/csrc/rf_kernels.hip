@@ -335,6 +335,135 @@ __device__ __forceinline__ void for_each_tile(bool valid, uint32_t r0, uint32_t 
     }
 }
 
+// Tile rectangle of an exact pixel rectangle (rows r0..r1, columns c0..c1).
+__device__ __forceinline__ void tile_rect_px(uint32_t r0, uint32_t r1, uint32_t c0, uint32_t c1, uint32_t tw,
+                                             uint32_t* out) {
+    out[0] = (c0 / tw) | ((r0 / TILE_H) << 16);
+    out[1] = (c1 / tw) | ((r1 / TILE_H) << 16);
+}
+
+// One Edge<usize> primitive (prim.rs:41-60): assembly, edge clipping
+// (clip.rs:311-348), to_screen, face culling (an edge is never a backface,
+// render.rs:72-75, so only FaceCull::Front drops it, ctx.rs:95-101), and the
+// set-up of raster::line (raster.rs:122-177). Returns the primitives kept (0
+// or 1). The ScrTri holds in v[0] the line's first end after the swaps (its z
+// and varyings are every fragment's) and in v[1] its pixel walk: first
+// column (wide) or row (tall), pixel count, the stepped coordinate's start
+// and step, and the wide flag. The binning rectangle is the exact pixel
+// rectangle; a pixel outside the target is the reference's out-of-bounds
+// panic (target.rs:173-174).
+__device__ uint32_t edge_setup(const BatchDesc& B, const DrawDesc* D, const FrameDesc& F, uint32_t di, uint64_t g,
+                               uint64_t t, uint32_t* rect, bool& has_rect) {
+    const uint32_t i0 = D->tris[2 * t], i1 = D->tris[2 * t + 1];
+    if (i0 >= D->n_verts || i1 >= D->n_verts) {
+        atomicOr(B.err, ERRB_INDEX);
+        return 0;
+    }
+    const uint64_t s0 = D->vert_base + i0, s1 = D->vert_base + i1;
+    const uint32_t o0 = f2u(B.ca[s0].w), o1 = f2u(B.ca[s1].w);
+    if (o0 & o1) return 0;  // both ends outside one plane
+    const bool persp = kind_persp(D->kind);
+    float a[6], b[6];
+    if ((o0 | o1) == 0) {
+        for (int c = 0; c < 6; ++c) {
+            a[c] = B.sv[6 * s0 + c];
+            b[c] = B.sv[6 * s1 + c];
+        }
+    } else {
+        CV ea, eb;
+        {
+            const float4 p0 = B.cv[s0], q0 = B.ca[s0], p1 = B.cv[s1], q1 = B.ca[s1];
+            const float pa[4] = {p0.x, p0.y, p0.z, p0.w}, aa[3] = {q0.x, q0.y, q0.z};
+            const float pb[4] = {p1.x, p1.y, p1.z, p1.w}, ab[3] = {q1.x, q1.y, q1.z};
+            cv_make(ea, pa, aa);
+            cv_make(eb, pb, ab);
+        }
+        for (int p = 0; p < 6; ++p) {
+            const uint32_t bit = 1u << p;
+            const bool a_in = (bit & ea.oc) == 0, b_in = (bit & eb.oc) == 0;
+            if (!a_in && !b_in) return 0;
+            // ClipPlane::intersect (clip.rs:121-145)
+            const float d0 = plane_dist(p, ea.p[0], ea.p[1], ea.p[2], ea.p[3]);
+            const float d1 = plane_dist(p, eb.p[0], eb.p[1], eb.p[2], eb.p[3]);
+            if (d0 * d1 < 0.0f) {
+                const float tt = -d0 / (d1 - d0);
+                float q[4], qa[3];
+                for (int c = 0; c < 4; ++c) q[c] = ea.p[c] + (eb.p[c] - ea.p[c]) * tt;
+                for (int c = 0; c < 3; ++c) qa[c] = ea.a[c] + (eb.a[c] - ea.a[c]) * tt;
+                CV v;
+                cv_make(v, q, qa);
+                if (a_in) eb = v;
+                else if (b_in) ea = v;
+            }
+        }
+        to_screen(ea, D->vp, persp, a);
+        to_screen(eb, D->vp, persp, b);
+    }
+    if (F.face_cull == FC_FRONT) return 0;
+    if (!(__builtin_isfinite(a[0]) && __builtin_isfinite(a[1]) && __builtin_isfinite(b[0]) &&
+          __builtin_isfinite(b[1]))) {
+        atomicOr(B.err, ERRB_NONFINITE);
+        return 0;
+    }
+    // raster::line: top end first; a wide line then runs left to right
+    float v0[6], v1[6];
+    const bool sw = a[1] > b[1];
+    for (int c = 0; c < 6; ++c) {
+        v0[c] = sw ? b[c] : a[c];
+        v1[c] = sw ? a[c] : b[c];
+    }
+    const float dx = v1[0] - v0[0], dy = v1[1] - v0[1];
+    const bool wide = fabsf(dx) > dy;
+    if (wide && dx < 0.0f)
+        for (int c = 0; c < 6; ++c) {
+            const float tmp = v0[c];
+            v0[c] = v1[c];
+            v1[c] = tmp;
+        }
+    // wide: columns (x0 as usize)..(x1 as usize), y stepped by dy/dx from its
+    // value at the rounded x0; tall: rows likewise, x stepped by dx/dy
+    const int ax = wide ? 0 : 1;  // the axis walked pixel by pixel
+    const float r0 = round_up_to_half(v0[ax]), r1 = round_up_to_half(v1[ax]);
+    const float step = wide ? dy / dx : dx / dy;
+    const float base = v0[1 - ax] + step * (r0 - v0[ax]);
+    const uint64_t S64 = as_usize(r0), E64 = as_usize(r1);
+    const uint64_t n64 = E64 > S64 ? E64 - S64 : 0;
+    float* d = &B.scr[g].v[0][0];
+    for (int c = 0; c < 6; ++c) d[c] = v0[c];
+    B.scr[g].draw = di;
+    B.scr[g].kind = D->kind | KIND_LINE;
+    if (n64 == 0) {
+        d[6] = u2f(0);
+        d[7] = u2f(0);
+        return 1;  // counted, draws nothing
+    }
+    const uint64_t lim_walk = wide ? F.width : F.height, lim_step = wide ? F.height : F.width;
+    if (E64 > lim_walk) {
+        atomicOr(B.err, ERRB_BOUNDS);
+        return 1;
+    }
+    const uint32_t S = (uint32_t)S64, n = (uint32_t)n64;
+    const float last = n > 1 ? ff_add(base, step, n - 1) : base;
+    // the stepped coordinate is monotone along the walk, so its pixels lie
+    // between those of the first and last steps
+    const uint64_t e0 = as_usize(base), e1 = as_usize(last);
+    const uint64_t lo = e0 < e1 ? e0 : e1, hi = e0 < e1 ? e1 : e0;
+    if (hi >= lim_step) {
+        atomicOr(B.err, ERRB_BOUNDS);
+        return 1;
+    }
+    d[6] = u2f(S);
+    d[7] = u2f(n);
+    d[8] = base;
+    d[9] = step;
+    d[10] = u2f(wide ? 1u : 0u);
+    d[11] = 0.0f;
+    if (wide) tile_rect_px((uint32_t)lo, (uint32_t)hi, S, S + n - 1, B.tile_w, rect);
+    else tile_rect_px(S, S + n - 1, (uint32_t)lo, (uint32_t)hi, B.tile_w, rect);
+    has_rect = true;
+    return 1;
+}
+
 // ---------------------------------------------------------------------------
 // k_setup
 // ---------------------------------------------------------------------------
@@ -354,12 +483,15 @@ __global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
         const FrameDesc& F = B.frames[D->frame];
         frame = D->frame;
         const uint64_t t = g - D->tri_base;
-        const uint32_t i0 = D->tris[3 * t], i1 = D->tris[3 * t + 1], i2 = D->tris[3 * t + 2];
         if (B.any_cull && !B.draw_vis[di]) {
             // the caller skips a hidden object before render() (crates.rs:118-122)
-        } else if (i0 >= D->n_verts || i1 >= D->n_verts || i2 >= D->n_verts) {
+        } else if (D->prim == PRIM_EDGE) {
+            nf = edge_setup(B, D, F, di, g, t, rect, has_rect);
+        } else if (D->tris[3 * t] >= D->n_verts || D->tris[3 * t + 1] >= D->n_verts ||
+                   D->tris[3 * t + 2] >= D->n_verts) {
             atomicOr(B.err, ERRB_INDEX);
         } else {
+            const uint32_t i0 = D->tris[3 * t], i1 = D->tris[3 * t + 1], i2 = D->tris[3 * t + 2];
             const uint64_t s0 = D->vert_base + i0, s1 = D->vert_base + i1, s2 = D->vert_base + i2;
             const uint32_t o0 = f2u(B.ca[s0].w), o1 = f2u(B.ca[s1].w), o2 = f2u(B.ca[s2].w);
             const uint32_t all_out = o0 & o1 & o2, any_out = o0 | o1 | o2;
@@ -692,10 +824,6 @@ struct Trap {
 struct Slot {
     Trap t[2];
     int32_t ylo, yhi;      // rows covered (absolute)
-    uint32_t kind, flat;
-    const uint8_t* tex;
-    uint32_t tw, th, tstride;
-    uint32_t rx0, rx1;     // binned tile columns (margin check)
 };
 
 // scan (raster.rs:248-302) for one trapezoid; y components are not needed by
@@ -778,59 +906,6 @@ __device__ __forceinline__ void tri_setup(Slot& S, float (&va)[6], float (&vb)[6
     }
     S.ylo = lo;
     S.yhi = hi;
-}
-
-// What the fragment stage needs of a triangle's draw.
-struct ShadeInfo {
-    uint32_t kind, flat;
-    const uint8_t* tex;
-    uint32_t tw, th, tstride, pad;
-};
-
-// Built-in fragment shader for one fragment, result in the target format.
-__device__ __forceinline__ uint32_t shade(const ShadeInfo& S, uint32_t fmt, float z, float a0, float a1,
-                                          float a2) {
-    uint32_t r, g, b, a;
-    switch (S.kind) {
-        case KIND_FLAT:
-            r = S.flat & 0xFF; g = (S.flat >> 8) & 0xFF; b = (S.flat >> 16) & 0xFF; a = S.flat >> 24;
-            break;
-        case KIND_GOURAUD:  // Color3f::to_color4 (color.rs:253-263)
-            r = as_u8(256.0f * a0); g = as_u8(256.0f * a1); b = as_u8(256.0f * a2); a = 0xFF;
-            break;
-        default: {
-            // Scanline::fragments z_div (raster.rs:61-68, vec.rs:666-676)
-            const float u = a0 / z, v = a1 / z;
-            const float tw = (float)S.tw, th = (float)S.th;
-            uint32_t ui, vi;
-            if (S.kind == KIND_TEX_CLAMP) {  // SamplerClamp (tex.rs:280-303)
-                float su = u * tw, sv = v * th;
-                const float mu = tw - 1.0f, mv = th - 1.0f;
-                if (su < 0.0f) su = 0.0f;
-                if (su > mu) su = mu;
-                if (sv < 0.0f) sv = 0.0f;
-                if (sv > mv) sv = mv;
-                ui = as_u32(floorf(su));
-                vi = as_u32(floorf(sv));
-            } else {  // SamplerRepeatPot (tex.rs:240-266)
-                ui = (uint32_t)as_i32(floorf(tw * u)) & (S.tw - 1);
-                vi = (uint32_t)as_i32(floorf(th * v)) & (S.th - 1);
-            }
-            const uint32_t t =
-                *(const uint32_t*)(S.tex + ((uint64_t)vi * S.tstride + ui) * 4ull);
-            r = t & 0xFF; g = (t >> 8) & 0xFF; b = (t >> 16) & 0xFF; a = t >> 24;
-        }
-    }
-    return pack_pixel(fmt, r, g, b, a);
-}
-
-__device__ __forceinline__ bool depth_pass(int32_t dt, float nz, float cur) {
-    switch (dt) {
-        case DT_NONE: return true;
-        case DT_LESS: return cur < nz;
-        case DT_EQUAL: return cur == nz;
-        default: return cur > nz;
-    }
 }
 
 __device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint32_t y, uint32_t v) {
@@ -1060,6 +1135,7 @@ constexpr int REFILL_MIN = RF_REFILL_MIN;  // free lanes that trigger a refill w
 // target.
 constexpr uint16_t TAG_NONE = 0xFFFFu;   // no fragment wrote colour
 constexpr uint16_t TAG_COLOR = 0xFFFEu;  // payload is the colour; else tag = draw - base, payload = texel
+constexpr uint32_t LINE_TAG = 0x8000u;   // SlotP::kind_tag bit of a line primitive
 
 __device__ __forceinline__ uint32_t texel_pixel(const BatchDesc& B, uint32_t fmt, uint32_t draw, uint32_t texel) {
     const uint32_t t = ((const uint32_t*)B.draws[draw].tex)[texel];
@@ -1252,6 +1328,11 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                 int32_t ra = 0, rb = 0;
                 uint32_t rx0b = 0, rx1b = 0;
                 int na = 0;
+                // a line (KIND_LINE): one record per pixel of its walk that lies in
+                // the tile's columns (wide) or rows (tall), from step lka on
+                bool is_line = false, lwide = false;
+                uint32_t lS = 0, lka = 0;
+                float lbase = 0.0f, lstep = 0.0f, lz = 0.0f, la0 = 0.0f, la1 = 0.0f, la2 = 0.0f;
                 if (act) {
                     const uint32_t e = s_fq[(fq_head + lane) % FQ_CAP];
                     const uint64_t gg = e >> 3;
@@ -1265,17 +1346,40 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                         vc[c] = st->v[2][c];
                     }
                     di = st->draw;
-                    kind = st->kind;
+                    kind = st->kind & 0xFFu;
+                    is_line = (st->kind & KIND_LINE) != 0u;
                     na = kind_attrs(kind);
-                    tri_setup(S, va, vb, vc, na);
-                    const uint32_t rr0 = B.rect[2 * gg], rr1 = B.rect[2 * gg + 1];
-                    rx0b = (rr0 & 0xFFFF) * TW;
-                    rx1b = ((rr1 & 0xFFFF) + 1) * TW;
-                    ra = S.ylo - (int32_t)y0;
-                    rb = S.yhi - (int32_t)y0;
-                    ra = ra < 0 ? 0 : ra;
-                    rb = rb > (int32_t)nrows ? (int32_t)nrows : rb;
-                    rows_in = rb > ra ? (uint32_t)(rb - ra) : 0u;
+                    if (is_line) {
+                        lS = f2u(vb[0]);
+                        const uint32_t ln = f2u(vb[1]);
+                        lbase = vb[2];
+                        lstep = vb[3];
+                        lwide = f2u(vb[4]) != 0u;
+                        lz = va[2];
+                        la0 = va[3];
+                        la1 = va[4];
+                        la2 = va[5];
+                        const uint32_t lo = lwide ? x0 : y0, hi = lo + (lwide ? ncols : nrows);
+                        lka = lo > lS ? lo - lS : 0u;
+                        const uint32_t kend = hi > lS ? min(ln, hi - lS) : 0u;
+                        rows_in = kend > lka ? kend - lka : 0u;
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            S.t[h].n = 0;
+                            S.t[h].dvz = 0.0f;
+                            for (int c = 0; c < 3; ++c) S.t[h].dva[c] = 0.0f;
+                        }
+                    } else {
+                        tri_setup(S, va, vb, vc, na);
+                        const uint32_t rr0 = B.rect[2 * gg], rr1 = B.rect[2 * gg + 1];
+                        rx0b = (rr0 & 0xFFFF) * TW;
+                        rx1b = ((rr1 & 0xFFFF) + 1) * TW;
+                        ra = S.ylo - (int32_t)y0;
+                        rb = S.yhi - (int32_t)y0;
+                        ra = ra < 0 ? 0 : ra;
+                        rb = rb > (int32_t)nrows ? (int32_t)nrows : rb;
+                        rows_in = rb > ra ? (uint32_t)(rb - ra) : 0u;
+                    }
                 }
                 uint32_t rec_tot;
                 const uint32_t rec_base = wave_excl_scan(rows_in, &rec_tot);
@@ -1319,7 +1423,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                         P.dv[h][0] = S.t[h].dvz;
                         for (int c = 0; c < 3; ++c) P.dv[h][1 + c] = S.t[h].dva[c];
                     }
-                    P.kind_tag = kind | ((di - dbase) << 16);
+                    P.kind_tag = kind | (is_line ? LINE_TAG : 0u) | ((di - dbase) << 16);
                     const DrawDesc& D = B.draws[di];
                     if (kind == KIND_FLAT) {
                         const uint32_t fc = D.flat_rgba;
@@ -1335,7 +1439,19 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                     if (rows_in) s_own[rec_base] = (int8_t)lane;
                 }
                 // ---- edges: lane = triangle, walk its rows in the tile
-                if (use && rows_in) {
+                if (use && rows_in && is_line) {
+                    // the line's pixels: the walked coordinate is the pixel
+                    // index, the stepped one goes on serially (raster.rs:138-175)
+                    float lv[1] = {lbase};
+                    const float ld[1] = {lstep};
+                    advance<1>(lv, ld, lka, KMAX, 1);
+                    EdgeRec* out = &s_rec[rec_base];
+                    for (uint32_t k = 0; k < rows_in; ++k) {
+                        const float pix = (float)(lS + lka + k);
+                        out[k] = lwide ? EdgeRec{pix, lv[0], lz, la0, la1, la2} : EdgeRec{lv[0], pix, lz, la0, la1, la2};
+                        lv[0] = lv[0] + lstep;
+                    }
+                } else if (use && rows_in) {
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         const Trap& T = S.t[h];
@@ -1384,10 +1500,34 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                 if (valid) {
                     const EdgeRec R = s_rec[k];
                     const SlotP& P = s_slot[f];
+                    if (P.kind_tag & LINE_TAG) {
+                        // a line pixel (Scanline x..x + 1 with the first end's
+                        // single varying value); only the tile holding it draws it
+                        const uint32_t px = cvt_u32(R.lx), py = cvt_u32(R.rx);
+                        if (px >= x0 && px < x0 + ncols && py >= y0 && py < y0 + nrows) {
+                            fi_acc += 1;
+                            ikind = P.kind_tag & 0xFFu;
+                            dz = d0 = d1 = d2 = 0.0f;
+                            z = R.lz;
+                            c0 = R.l0;
+                            c1 = R.l1;
+                            c2 = R.l2;
+                            have = true;
+                            key = key_base + k;
+                            row = py - y0;
+                            xs = (int32_t)px;
+                            jc = 0;
+                            j1 = 1;
+                            ip0 = P.p0;
+                            ip1 = P.p1;
+                            ip2 = P.p2;
+                            itag = P.kind_tag >> 16;
+                        }
+                    } else {
                     const int32_t r = (int32_t)(P.ra_rec & 0xFFFFu) + (int32_t)(k - (P.ra_rec >> 16));
                     const int32_t y = (int32_t)y0 + r;
                     const int h = y < P.ysplit ? 0 : 1;
-                    ikind = P.kind_tag & 0xFFFFu;
+                    ikind = P.kind_tag & 0xFFu;
                     dz = P.dv[h][0];
                     d0 = P.dv[h][1];
                     d1 = P.dv[h][2];
@@ -1437,6 +1577,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                         ip1 = P.p1;
                         ip2 = P.p2;
                         itag = P.kind_tag >> 16;
+                    }
                     }
                 }
             }

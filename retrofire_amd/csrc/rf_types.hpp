@@ -32,6 +32,16 @@ enum ShaderKind : uint32_t {
     KIND_TEX_REPEAT_POT = 3,  // TexCoord varying (z-divided) -> SamplerRepeatPot
 };
 
+// Primitive types of a draw (Render impls, prim.rs:11-60).
+enum PrimType : uint32_t {
+    PRIM_TRI = 0,   // Tri<usize>: index triples, tri_fill
+    PRIM_EDGE = 1,  // Edge<usize>: index pairs, raster::line
+};
+// ScrTri::kind bit of a line (Edge) primitive: then v[0] is the line's first
+// end (its z and varyings hold for every pixel) and v[1] its pixel walk
+// (see k_setup).
+constexpr uint32_t KIND_LINE = 0x100;
+
 __host__ __device__ inline int kind_attrs(uint32_t k) {
     return k == KIND_FLAT ? 0 : (k == KIND_GOURAUD ? 3 : 2);
 }
@@ -71,7 +81,7 @@ enum ErrBits : uint32_t {
 struct DrawDesc {
     const float* pos;       // n_verts x (x, y, z)
     const float* attr;      // n_verts x kind_attrs(kind)
-    const uint32_t* tris;   // n_tris x (i0, i1, i2)
+    const uint32_t* tris;   // n_tris x (i0, i1, i2), or x (i0, i1) for edges
     const uint8_t* tex;     // Color4 texels (textured kinds)
     uint32_t n_verts, n_tris;
     uint32_t kind, frame;
@@ -79,6 +89,8 @@ struct DrawDesc {
     uint32_t flat_rgba;                 // r | g << 8 | b << 16 | a << 24
     uint32_t attr_stride;               // floats per vertex in `attr`
     uint32_t cull;                      // 1: skipped when BBox::visibility(mvp) is Hidden (scene.rs:81-87)
+    uint32_t prim;                      // PrimType; n_tris counts primitives of that type
+    uint32_t pad1;
     uint64_t vert_base;                 // first slot in the batch vertex buffers
     uint64_t tri_base;                  // first global triangle index
     float mvp[16];                      // row-major ProjMat3 (the uniform)

@@ -72,7 +72,7 @@ def run_both(kind, tris, pos, attr, mvp, vp, w, h, fmt=rf.PIX_RGBA8, with_depth=
     verts = (pos, attr) if attr is not None else pos
     for i in range(calls):
         err, _ = O.render(kind, tris, pos, attr, mvp, vp, ocol, depth=odep, fmt=fmt, flat=flat, tex=tex,
-                          ctx=octx, stats=ost)
+                          ctx=octx, stats=ost, prim=1 if np.asarray(tris).shape[-1] == 2 else 0)
         assert err == 0, f"oracle error {err}"
         rf.render(tris, verts, shd, mvp, vp, target, gctx)
         gctx.clear = False  # a frame clears once
