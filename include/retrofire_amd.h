@@ -52,7 +52,10 @@ typedef enum rf_shader_kind {
     RF_SHADER_FLAT = 0,            /* |_| color: constant Color4, varying ()            */
     RF_SHADER_GOURAUD = 1,         /* Color3f varying, frag.var.to_color4()             */
     RF_SHADER_TEX_CLAMP = 2,       /* TexCoord varying, SamplerClamp.sample (tex.rs:272) */
-    RF_SHADER_TEX_REPEAT_POT = 3   /* TexCoord varying, SamplerRepeatPot (tex.rs:218)    */
+    RF_SHADER_TEX_REPEAT_POT = 3,  /* TexCoord varying, SamplerRepeatPot (tex.rs:218)    */
+    RF_SHADER_TEX_ONCE = 4         /* TexCoord varying, SamplerOnce (tex.rs:313-357): a
+                                    * texel outside the texture is RF_E_BOUNDS, where
+                                    * the reference's indexing panics (buf.rs:731-733) */
 } rf_shader_kind;
 
 /* Colour buffer element layouts (target.rs:35-136, pixfmt.rs:58-138). The
@@ -180,6 +183,11 @@ rf_status rf_mesh_create_edges(rf_device* dev, const float* pos, const float* at
 rf_status rf_mesh_free(rf_device* dev, rf_mesh* mesh);
 rf_status rf_texture_create(rf_device* dev, const uint8_t* rgba, uint32_t w, uint32_t h,
                             uint32_t stride_texels, rf_texture** out);
+/* A sub-texture of a resident texture, sharing its texels and row stride:
+ * Atlas::get (tex.rs:157-172) for a Grid layout cell, or any rectangle. The
+ * view must be freed before its texture. */
+rf_status rf_texture_view(rf_device* dev, const rf_texture* tex, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                          rf_texture** out);
 rf_status rf_texture_free(rf_device* dev, rf_texture* tex);
 /* A device-resident target; rf_framebuf_target fills an rf_target with its
  * device pointers. */

@@ -853,5 +853,24 @@ struct FsTexClamp {
         return true;
     }
 };
+// SamplerOnce::sample (tex.rs:327-357): uv scaled by the texture size, then
+// `as u32`; indexing outside the texture panics (Slice2 index, buf.rs:731-733
+// via to_index_strict buf.rs:444-457), recorded in `oob` (the call fails).
+struct FsTexOnce {
+    Texture tex;
+    bool* oob;
+    bool operator()(float, const Attr<2>& uv, Rgba& out) const {
+        const float tw = (float)tex.w, th = (float)tex.h;
+        const uint32_t u = as_u32(tw * uv.c[0]);
+        const uint32_t v = as_u32(th * uv.c[1]);
+        if (u >= tex.w || v >= tex.h) {
+            *oob = true;
+            out = Rgba{0, 0, 0, 0};
+            return true;
+        }
+        out = texel(tex, u, v);
+        return true;
+    }
+};
 
 }  // namespace oracle

@@ -12,7 +12,7 @@ using namespace oracle;
 extern "C" {
 
 // Shader kinds, same numbering as the product header.
-enum { K_FLAT = 0, K_GOURAUD = 1, K_TEX_CLAMP = 2, K_TEX_REPEAT_POT = 3 };
+enum { K_FLAT = 0, K_GOURAUD = 1, K_TEX_CLAMP = 2, K_TEX_REPEAT_POT = 3, K_TEX_ONCE = 4 };
 
 struct OracleCtx {
     int32_t depth_test;   // 0 none, 1 less, 2 equal, 3 greater
@@ -73,6 +73,15 @@ int oracle_render_prims(int prim, int kind, const uint32_t* tris, uint64_t n_tri
             FsTexRepeatPot fs{Texture{tex, tex_w, tex_h, tex_stride}};
             err = prim ? render<2, true, FsTexRepeatPot, 2>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st)
                        : render<2, true>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st);
+            break;
+        }
+        case K_TEX_ONCE: {
+            if (tex_w == 0 || tex_h == 0) return E_ARG;
+            bool oob = false;
+            FsTexOnce fs{Texture{tex, tex_w, tex_h, tex_stride}, &oob};
+            err = prim ? render<2, true, FsTexOnce, 2>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st)
+                       : render<2, true>(tris, n_tris, pos, attr, n_verts, mvp, viewport, fs, t, ctx, st);
+            if (err == OK && oob) return E_BOUNDS;
             break;
         }
         default:
@@ -357,7 +366,11 @@ int oracle_sample(int kind, const uint8_t* tex, uint32_t w, uint32_t h, uint32_t
         FsTexClamp{t}(1.0f, uv, c);
     else if (kind == K_TEX_REPEAT_POT)
         FsTexRepeatPot{t}(1.0f, uv, c);
-    else
+    else if (kind == K_TEX_ONCE) {
+        bool oob = false;
+        FsTexOnce{t, &oob}(1.0f, uv, c);
+        if (oob) return E_BOUNDS;
+    } else
         return E_ARG;
     out[0] = c.r; out[1] = c.g; out[2] = c.b; out[3] = c.a;
     return OK;

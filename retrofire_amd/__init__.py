@@ -118,6 +118,15 @@ class shader:
         def __init__(self, texture: np.ndarray):
             self.texture = np.ascontiguousarray(texture, dtype=np.uint8)
 
+    class TexOnce:
+        """SamplerOnce (tex.rs:313-357): texture coordinates must stay in
+        0..1; a texel outside the texture raises RenderError (RF_E_BOUNDS)."""
+        kind = 4
+        n_attr = 2
+
+        def __init__(self, texture: np.ndarray):
+            self.texture = np.asarray(texture, dtype=np.uint8)
+
 
 class Colorbuf:
     """Colour-only target: `color` is (h, w, 4) uint8 (RGBA8), (h, w, 3) uint8
@@ -228,6 +237,8 @@ def make_draw(prims, verts, shd, uniform, to_screen, keep: list) -> _ffi.RfDraw:
             d.flat_color[i] = shd.color[i]
     if shd.kind >= 2:
         tex = shd.texture
+        # a view into a larger texture (an atlas cell) keeps its row stride
+        assert tex.ndim == 3 and tex.shape[2] == 4 and tex.strides[2] == 1 and tex.strides[1] == 4
         d.tex = tex.ctypes.data_as(C.POINTER(C.c_uint8))
         d.tex_h, d.tex_w = tex.shape[0], tex.shape[1]
         d.tex_stride = tex.strides[0] // 4
@@ -243,7 +254,7 @@ def render(prims, verts, shd, uniform, to_screen, target, ctx: Context, device: 
     `uniform` is the mvp matrix of the built-in vertex shader; `to_screen` the
     viewport matrix. The target's arrays are updated in place; ctx.stats
     accumulates the call's statistics."""
-    if not isinstance(shd, (shader.Flat, shader.Gouraud, shader.TexClamp, shader.TexRepeatPot)):
+    if not isinstance(shd, (shader.Flat, shader.Gouraud, shader.TexClamp, shader.TexRepeatPot, shader.TexOnce)):
         raise RenderError(-5)
     dev = device or default_device()
     keep: list = []

@@ -79,6 +79,7 @@ SIGNATURES = {
     "rf_mesh_create_edges": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.c_uint64,
                                    C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(_vp)]),
     "rf_mesh_free": (_st, [_vp, _vp]),
+    "rf_texture_view": (_st, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
     "rf_texture_create": (_st, [_vp, C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.POINTER(_vp)]),
     "rf_texture_free": (_st, [_vp, _vp]),

@@ -42,12 +42,21 @@ class Mesh:
 
 
 class Texture:
-    def __init__(self, rgba: np.ndarray, device=None):
+    def __init__(self, rgba: np.ndarray | None, device=None):
         self.dev = device or default_device()
-        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
         self.h = C.c_void_p()
+        if rgba is None:
+            return
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
         check(_ffi.lib().rf_texture_create(self.dev.h, rgba.ctypes.data_as(C.POINTER(C.c_uint8)), rgba.shape[1],
                                            rgba.shape[0], rgba.shape[1], C.byref(self.h)))
+
+    def view(self, x, y, w, h):
+        """The sub-texture of the rectangle at (x, y), w x h texels, sharing
+        this texture's memory (Atlas::get, tex.rs:157-172). Free it first."""
+        v = Texture(None, self.dev)
+        check(_ffi.lib().rf_texture_view(self.dev.h, self.h, x, y, w, h, C.byref(v.h)))
+        return v
 
     def free(self):
         if self.h:

@@ -72,7 +72,7 @@ rf_status check_ctx(const rf_ctx* c) {
 
 rf_status check_tex(uint32_t kind, const void* tex, uint32_t w, uint32_t h, uint32_t stride) {
     if (kind == KIND_FLAT || kind == KIND_GOURAUD) return RF_OK;
-    if (kind > KIND_TEX_REPEAT_POT) return RF_E_UNSUPPORTED;
+    if (kind > KIND_LAST) return RF_E_UNSUPPORTED;
     if (!tex || w == 0 || h == 0 || stride < w) return RF_E_ARG;
     if ((uint64_t)stride * h > 0xFFFFFFFFull) return RF_E_ARG;  // texel indices are 32-bit
     if (kind == KIND_TEX_REPEAT_POT && ((w & (w - 1)) || (h & (h - 1)))) return RF_E_ARG;
@@ -81,7 +81,7 @@ rf_status check_tex(uint32_t kind, const void* tex, uint32_t w, uint32_t h, uint
 
 rf_status map_err(uint32_t e) {
     if (e & ERRB_INDEX) return RF_E_INDEX;
-    if (e & ERRB_BOUNDS) return RF_E_BOUNDS;
+    if (e & (ERRB_BOUNDS | ERRB_TEXEL)) return RF_E_BOUNDS;
     if (e & ERRB_NEGY) return RF_E_VIEWPORT;
     if (e & ERRB_NONFINITE) return RF_E_NONFINITE;
     if (e & (ERRB_MARGIN | ERRB_EXTRA | ERRB_RUNS | ERRB_CLIP)) return RF_E_INTERNAL;
@@ -276,7 +276,7 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     Device& d = *reinterpret_cast<Device*>(dev);
     rf_status st = check_ctx(ctx);
     if (st) return st;
-    if (draw->kind > KIND_TEX_REPEAT_POT) return RF_E_UNSUPPORTED;
+    if (draw->kind > KIND_LAST) return RF_E_UNSUPPORTED;
     if (draw->prim > PRIM_EDGE) return RF_E_ARG;
     const uint32_t ipp = draw->prim == PRIM_EDGE ? 2u : 3u;  // indices per primitive
     if ((st = check_tex(draw->kind, draw->tex, draw->tex_w, draw->tex_h, draw->tex_stride))) return st;
@@ -488,11 +488,27 @@ rf_status rf_texture_create(rf_device* dev, const uint8_t* rgba, uint32_t w, uin
     return RF_OK;
 }
 
+rf_status rf_texture_view(rf_device* dev, const rf_texture* tex, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                          rf_texture** out) {
+    if (!dev || !tex || !out || !w || !h) return RF_E_ARG;
+    const Tex* p = reinterpret_cast<const Tex*>(tex);
+    if ((uint64_t)x + w > p->w || (uint64_t)y + h > p->h) return RF_E_ARG;  // Slice2 bounds (buf.rs:459-480)
+    Tex* t = new (std::nothrow) Tex();
+    if (!t) return RF_E_DEVICE;
+    t->parent = p;
+    t->offset = ((uint64_t)y * p->stride + x) * 4;
+    t->w = w;
+    t->h = h;
+    t->stride = p->stride;
+    *out = reinterpret_cast<rf_texture*>(t);
+    return RF_OK;
+}
+
 rf_status rf_texture_free(rf_device* dev, rf_texture* tex) {
     if (!dev || !tex) return RF_E_ARG;
     Device& d = *reinterpret_cast<Device*>(dev);
     Tex* t = reinterpret_cast<Tex*>(tex);
-    t->data.release(&d.bytes);
+    t->data.release(&d.bytes);  // a view owns nothing
     delete t;
     return RF_OK;
 }

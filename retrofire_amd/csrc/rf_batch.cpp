@@ -74,7 +74,7 @@ static rf_status build_descs(Batch& b, uint32_t tile_w) {
             dd.attr = (const float*)m->attr.p;
             dd.tris = (const uint32_t*)m->tris.p + (m->prim == PRIM_EDGE ? 2 : 3) * first;
             dd.prim = m->prim;
-            dd.tex = t ? (const uint8_t*)t->data.p : nullptr;
+            dd.tex = t ? t->texels() : nullptr;
             dd.n_verts = (uint32_t)m->n_verts;
             dd.n_tris = (uint32_t)nt;
             dd.kind = md.kind;
@@ -197,12 +197,12 @@ rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_fra
         if (st) return st;
         for (uint32_t k = 0; k < f.n_draws; ++k) {
             const rf_mesh_draw& md = f.draws[k];
-            if (!md.mesh || md.kind > KIND_TEX_REPEAT_POT || md.cull > 1) return RF_E_ARG;
+            if (!md.mesh || md.kind > KIND_LAST || md.cull > 1) return RF_E_ARG;
             const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
             const Tex* t = reinterpret_cast<const Tex*>(md.tex);
             if ((int)m->n_attr < kind_attrs(md.kind)) return RF_E_ARG;
             if (md.first_tri > m->n_tris || md.first_tri + md.n_tris > m->n_tris) return RF_E_ARG;
-            if ((st = check_tex(md.kind, t ? t->data.p : nullptr, t ? t->w : 0, t ? t->h : 0, t ? t->stride : 0)))
+            if ((st = check_tex(md.kind, t ? t->texels() : nullptr, t ? t->w : 0, t ? t->h : 0, t ? t->stride : 0)))
                 return st;
         }
     }

@@ -57,8 +57,11 @@ struct Mesh {
 };
 
 struct Tex {
-    DevBuf data;
-    uint32_t w = 0, h = 0, stride = 0;
+    DevBuf data;                        // owned texels (empty for a view)
+    const Tex* parent = nullptr;        // a view's texture (Atlas::get, tex.rs:163-172)
+    uint64_t offset = 0;                // a view's first texel in its parent, in bytes
+    uint32_t w = 0, h = 0, stride = 0;  // stride in texels
+    const uint8_t* texels() const { return parent ? parent->texels() + offset : (const uint8_t*)data.p; }
 };
 
 struct Framebuf {

@@ -1628,6 +1628,14 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                     ps[p] = (p < rem) & depth_pass_u(dtest, zz[p], cur[p]);
                     npass += ps[p] ? 1u : 0u;
                 }
+                if (ikind == KIND_TEX_ONCE) {
+                    // the shader runs for every passing fragment, colour write or not
+                    const float tw = (float)ip0, th = (float)ip1;
+#pragma unroll
+                    for (int p = 0; p < CHUNK; ++p)
+                        if (ps[p] && (cvt_u32(tw * (v0[p] / zz[p])) >= ip0 || cvt_u32(th * (v1[p] / zz[p])) >= ip1))
+                            errs |= ERRB_TEXEL;
+                }
                 if (F.color_write) {
                     fo_acc += npass;
                     // colour sources of the chunk's pixels (the shader kind is
@@ -1659,12 +1667,21 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                                 sv = sv > mv ? mv : sv;
                                 pay[p] = cvt_u32(floorf(sv)) * ip2 + cvt_u32(floorf(su));
                             }
-                        } else {  // SamplerRepeatPot (tex.rs:240-266)
+                        } else if (ikind == KIND_TEX_REPEAT_POT) {  // SamplerRepeatPot (tex.rs:240-266)
 #pragma unroll
                             for (int p = 0; p < CHUNK; ++p) {
                                 const uint32_t ui = (uint32_t)cvt_i32(floorf(tw * (v0[p] / zz[p]))) & (ip0 - 1);
                                 const uint32_t vi = (uint32_t)cvt_i32(floorf(th * (v1[p] / zz[p]))) & (ip1 - 1);
                                 pay[p] = vi * ip2 + ui;
+                            }
+                        } else {  // SamplerOnce (tex.rs:343-357): `as u32`, no wrap
+#pragma unroll
+                            for (int p = 0; p < CHUNK; ++p) {
+                                const uint32_t ui = cvt_u32(tw * (v0[p] / zz[p]));
+                                const uint32_t vi = cvt_u32(th * (v1[p] / zz[p]));
+                                // outside the texture the reference panics (checked
+                                // below); never fetch there
+                                pay[p] = (ui < ip0 && vi < ip1) ? vi * ip2 + ui : 0u;
                             }
                         }
                     }

@@ -260,7 +260,7 @@ static rf_status launch_lane(Serve& s, Lane& l) {
             dd.attr = (const float*)m->attr.p;
             dd.tris = (const uint32_t*)m->tris.p + (m->prim == PRIM_EDGE ? 2 : 3) * md.first_tri;
             dd.prim = m->prim;
-            dd.tex = t ? (const uint8_t*)t->data.p : nullptr;
+            dd.tex = t ? t->texels() : nullptr;
             dd.n_verts = (uint32_t)m->n_verts;
             dd.n_tris = (uint32_t)nt;
             dd.kind = md.kind;
@@ -493,12 +493,12 @@ rf_status rf_serve_submit(rf_serve* sh, const rf_request* req) {
     q.draws.assign(req->draws, req->draws + req->n_draws);
     q.r.draws = nullptr;
     for (const rf_mesh_draw& md : q.draws) {
-        if (!md.mesh || md.kind > KIND_TEX_REPEAT_POT || md.cull > 1) return RF_E_ARG;
+        if (!md.mesh || md.kind > KIND_LAST || md.cull > 1) return RF_E_ARG;
         const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
         const Tex* t = reinterpret_cast<const Tex*>(md.tex);
         if ((int)m->n_attr < kind_attrs(md.kind)) return RF_E_ARG;
         if (md.first_tri > m->n_tris || md.first_tri + md.n_tris > m->n_tris) return RF_E_ARG;
-        if ((st = check_tex(md.kind, t ? t->data.p : nullptr, t ? t->w : 0, t ? t->h : 0, t ? t->stride : 0)))
+        if ((st = check_tex(md.kind, t ? t->texels() : nullptr, t ? t->w : 0, t ? t->h : 0, t ? t->stride : 0)))
             return st;
         q.n_tris += md.n_tris ? md.n_tris : m->n_tris - md.first_tri;
         q.n_verts += m->n_verts;

@@ -30,7 +30,9 @@ enum ShaderKind : uint32_t {
     KIND_GOURAUD = 1,         // Color3f varying -> Color3f::to_color4
     KIND_TEX_CLAMP = 2,       // TexCoord varying (z-divided) -> SamplerClamp
     KIND_TEX_REPEAT_POT = 3,  // TexCoord varying (z-divided) -> SamplerRepeatPot
+    KIND_TEX_ONCE = 4,        // TexCoord varying (z-divided) -> SamplerOnce
 };
+constexpr uint32_t KIND_LAST = KIND_TEX_ONCE;
 
 // Primitive types of a draw (Render impls, prim.rs:11-60).
 enum PrimType : uint32_t {
@@ -76,6 +78,7 @@ enum ErrBits : uint32_t {
     ERRB_NONFINITE = 1u << 5,  // a screen-space vertex is NaN or infinite
     ERRB_NEGY = 1u << 6,       // a screen-space vertex lies above row -0.5
     ERRB_CLIP = 1u << 7,       // clipping produced more than 9 vertices
+    ERRB_TEXEL = 1u << 8,      // SamplerOnce indexed outside the texture (buf.rs:731-733 panics)
 };
 
 struct DrawDesc {

@@ -63,8 +63,10 @@ def run_both(kind, tris, pos, attr, mvp, vp, w, h, fmt=rf.PIX_RGBA8, with_depth=
         shd = rf.shader.Gouraud()
     elif kind == 2:
         shd = rf.shader.TexClamp(tex)
-    else:
+    elif kind == 3:
         shd = rf.shader.TexRepeatPot(tex)
+    else:
+        shd = rf.shader.TexOnce(tex)
     gctx = rf.Context(color_clear=ctx.color_clear, depth_clear=ctx.depth_clear, face_cull=ctx.face_cull,
                       depth_test=ctx.depth_test, color_write=ctx.color_write, depth_write=ctx.depth_write,
                       clear=ctx.clear)

@@ -415,6 +415,18 @@ def test_sampler_clamp():  # tex.rs:395-406
     assert s(1.5, 1.5) == Y
 
 
+def test_sampler_once():  # tex.rs:408-417
+    s = lambda u, v: _sample(4, u, v)
+    assert s(0.0, 0.0) == R and s(0.5, 0.0) == G
+    assert s(0.0, 0.5) == B and s(0.5, 0.5) == Y
+    # `as u32` truncates: just below 1.0 is the last texel; 1.0 and beyond is
+    # outside the texture, where the reference's indexing panics (buf.rs:731-733)
+    assert s(0.99, 0.99) == Y and s(-0.4, 0.0) == R
+    for u, v in ((1.0, 0.0), (0.0, 1.0), (7.0, 0.2)):
+        with pytest.raises(AssertionError):
+            s(u, v)
+
+
 # ------------------------------------------------------- golden images
 
 def read_ppm(path):
