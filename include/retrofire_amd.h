@@ -41,7 +41,8 @@ typedef int32_t rf_status;
                                  SamplerRepeatPot, tex.rs:228-233 panics)            */
 #define RF_E_DEVICE (-4)      /* HIP runtime error or out of device memory         */
 #define RF_E_UNSUPPORTED (-5) /* not a built-in shader kind / target layout, or
-                                 depth_sort set: the caller keeps the CPU path      */
+                                 depth_sort on the batched or serving path: the
+                                 caller keeps the CPU path                          */
 #define RF_E_VIEWPORT (-6)    /* to_screen maps the NDC cube outside the target   */
 #define RF_E_NONFINITE (-7)   /* a screen-space vertex is NaN or infinite          */
 #define RF_E_INTERNAL (-8)    /* internal capacity or binning check failed         */
@@ -75,8 +76,12 @@ typedef enum rf_pixfmt {
 
 /* Context (ctx.rs:11-65). depth_test holds the reference's
  * Option<Ordering>: 0 = None, 1 = Less, 2 = Equal, 3 = Greater. face_cull:
- * 0 = None, 1 = Back, 2 = Front. depth_sort must be 0 (None); other values
- * return RF_E_UNSUPPORTED. clear_color / clear_depth fuse the frontend's
+ * 0 = None, 1 = Back, 2 = Front. depth_sort (Option<DepthSort>, ctx.rs:68-72):
+ * 0 = None, 1 = FrontToBack, 2 = BackToFront; rf_render sorts the clipped
+ * triangles by Render::depth under f32::total_cmp (render.rs:209-219), equal
+ * depths in submission order (the reference's sort_unstable_by leaves their
+ * order to the Rust toolchain); the batched and serving paths return
+ * RF_E_UNSUPPORTED for it. clear_color / clear_depth fuse the frontend's
  * Frame::clear (front/src/lib.rs:103-120) into the call: the colour buffer is
  * first filled with color_clear, the depth buffer with 1 / depth_clear. */
 typedef struct rf_ctx {

@@ -510,6 +510,7 @@ struct Ctx {
     int face_cull = FC_BACK;   // ctx.rs:119
     bool color_write = true;
     bool depth_write = true;
+    int depth_sort = 0;        // 0 None, 1 FrontToBack, 2 BackToFront (ctx.rs:39, 68-72)
 };
 
 // Context::depth_test (ctx.rs:86-89): `curr.partial_cmp(&new) == depth_test`.
@@ -768,6 +769,19 @@ inline int render(const uint32_t* tris, uint64_t n_tris, const float* pos, const
     std::vector<TriC<N>> clipped;
     clipped.reserve(n_tris / 2);
     clip_tris<N>(prims, clipped);
+    // optional depth sort (render.rs:180-183, 209-219) by Render::depth
+    // ((a.z + b.z + c.z) / 3.0 in clip space, prim.rs:21-23) under
+    // f32::total_cmp, ascending front to back, descending back to front. The
+    // reference's sort_unstable_by leaves the order of equal keys to the Rust
+    // toolchain's sort; this restatement keeps them in submission order.
+    if (ctx.depth_sort) {
+        auto depth = [](const TriC<N>& t) { return ((t.v[0].pos[2] + t.v[1].pos[2]) + t.v[2].pos[2]) / 3.0f; };
+        const bool back = ctx.depth_sort == 2;
+        std::stable_sort(clipped.begin(), clipped.end(), [&](const TriC<N>& a, const TriC<N>& b) {
+            const int32_t ka = total_key(depth(a)), kb = total_key(depth(b));
+            return back ? kb < ka : ka < kb;
+        });
+    }
 
     // 4. per primitive (render.rs:185-205)
     int err = OK;

@@ -19,6 +19,7 @@ struct OracleCtx {
     int32_t face_cull;    // 0 none, 1 back, 2 front
     int32_t color_write;
     int32_t depth_write;
+    int32_t depth_sort;   // 0 none, 1 front to back, 2 back to front
 };
 
 // One render() call. Stats are accumulated into stats[0..7]:
@@ -45,6 +46,8 @@ int oracle_render_prims(int prim, int kind, const uint32_t* tris, uint64_t n_tri
     ctx.face_cull = c->face_cull;
     ctx.color_write = c->color_write != 0;
     ctx.depth_write = c->depth_write != 0;
+    ctx.depth_sort = c->depth_sort;
+    if (ctx.depth_sort < 0 || ctx.depth_sort > 2) return E_ARG;
     Stats st;
     int err = E_ARG;
     switch (kind) {

@@ -25,6 +25,7 @@ PIX_RGBA8, PIX_RGB8, PIX_U32_RGBA, PIX_U32_XRGB = 0, 1, 2, 3
 PIX_U32_ARGB, PIX_U32_BGRA, PIX_RGBA4444, PIX_RGB565 = 4, 5, 6, 7
 _DT = {None: 0, "less": 1, "equal": 2, "greater": 3}
 _FC = {None: 0, "back": 1, "front": 2}
+_DS = {None: 0, "front_to_back": 1, "back_to_front": 2}
 
 
 class RenderError(RuntimeError):
@@ -80,7 +81,7 @@ class Context:
         c.face_cull = _FC[self.face_cull]
         c.color_write = int(self.color_write)
         c.depth_write = int(self.depth_write)
-        c.depth_sort = 0 if self.depth_sort is None else 1
+        c.depth_sort = _DS[self.depth_sort]
         c.clear_color = int(self.clear and self.color_clear is not None)
         cc = self.color_clear or (0, 0, 0, 0)
         for i in range(4):

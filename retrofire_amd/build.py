@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librfamd.so")
-SOURCES = ["rf_kernels.hip", "rf_api.cpp", "rf_batch.cpp", "rf_serve.cpp"]
+SOURCES = ["rf_kernels.hip", "rf_sort.hip", "rf_api.cpp", "rf_batch.cpp", "rf_serve.cpp"]
 HEADERS = ["rf_types.hpp", "rf_math.hpp", "rf_internal.hpp"]
 ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")
 

@@ -65,7 +65,7 @@ void fill_frame_ctx(FrameDesc& f, const rf_ctx& c) {
 
 rf_status check_ctx(const rf_ctx* c) {
     if (!c) return RF_E_ARG;
-    if (c->depth_sort != 0) return RF_E_UNSUPPORTED;
+    if (c->depth_sort < 0 || c->depth_sort > 2) return RF_E_ARG;
     if (c->depth_test < 0 || c->depth_test > 3 || c->face_cull < 0 || c->face_cull > 2) return RF_E_ARG;
     return RF_OK;
 }
@@ -250,7 +250,7 @@ rf_status rf_device_close(rf_device* dev) {
     (void)hipStreamSynchronize(d->stream);
     d->plain.release(&d->bytes);
     for (DevBuf* b : {&d->up_pos, &d->up_attr, &d->up_tris, &d->up_tex, &d->up_color, &d->up_depth,
-                      &d->up_desc})
+                      &d->up_desc, &d->ds_fans, &d->ds_keys, &d->ds_vals, &d->ds_tmp, &d->ds_idx})
         b->release(&d->bytes);
     (void)hipStreamDestroy(d->stream);
     delete d;
@@ -372,6 +372,67 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     B.tile_h = TILE_ROWS;
     B.extra_cap = std::max<uint32_t>(d.plain_extra_cap, 1024u);
     B.runs_cap = std::max<uint32_t>(d.plain_runs_cap, 4 * B.n_groups + B.n_tiles + 1024);
+
+    // Depth sort (render.rs:180-183): the clip fans are sorted by depth on the
+    // device and drawn in that order as unclipped triangles. Edges keep their
+    // order: Render::depth is +inf for every edge (render.rs:67-70), and
+    // sort_unstable_by leaves a slice of equal keys as it is.
+    if (ctx->depth_sort && draw->prim == PRIM_TRI && draw->n_tris) {
+        if (draw->n_tris >= (1ull << 29)) return RF_E_ARG;  // triangle << 3 | fan must fit 32 bits
+        uint32_t cap = std::max<uint32_t>(d.ds_cap, (uint32_t)std::min<uint64_t>(2 * draw->n_tris + 1024, 1u << 30));
+        uint32_t n_f = 0;
+        for (int attempt = 0;; ++attempt) {
+            if (attempt == 4) return RF_E_INTERNAL;
+            if (d.plain.prepare(B, &d.bytes) || d.ds_fans.ensure((size_t)cap * FAN_FLOATS * 4, &d.bytes) ||
+                d.ds_keys.ensure((size_t)cap * 16, &d.bytes) || d.ds_vals.ensure((size_t)cap * 8, &d.bytes))
+                return RF_E_DEVICE;
+            BatchDesc A = B;
+            A.depth_sort = (uint32_t)ctx->depth_sort;
+            A.fan_cap = cap;
+            A.fans = (float*)d.ds_fans.p;
+            A.fan_keys = (unsigned long long*)d.ds_keys.p;
+            A.fan_vals = (uint32_t*)d.ds_vals.p;
+            A.fan_count = (uint32_t*)d.plain.misc.p + 32;
+            uint32_t misc[33];
+            if (launch_fans(A, s) ||
+                hipMemcpyAsync(misc, d.plain.misc.p, sizeof(misc), hipMemcpyDeviceToHost, s) ||
+                hipStreamSynchronize(s))
+                return RF_E_DEVICE;
+            const uint32_t e = misc[2];
+            if (e & ERRB_INDEX) return RF_E_INDEX;
+            if (e & ERRB_CLIP) return RF_E_INTERNAL;
+            if (e & ERRB_EXTRA) {
+                cap = misc[32] + misc[32] / 4 + 1024;
+                continue;
+            }
+            n_f = misc[32];
+            break;
+        }
+        d.ds_cap = cap;
+        unsigned long long* keys = (unsigned long long*)d.ds_keys.p;
+        uint32_t* vals = (uint32_t*)d.ds_vals.p;
+        size_t tb = 0;
+        if (sort_pairs_u64(nullptr, tb, keys, keys + cap, vals, vals + cap, n_f, s) ||
+            d.ds_tmp.ensure(std::max<size_t>(tb, 4), &d.bytes) ||
+            sort_pairs_u64(d.ds_tmp.p, tb, keys, keys + cap, vals, vals + cap, n_f, s) ||
+            d.ds_idx.ensure(std::max<size_t>((size_t)n_f * 12, 4), &d.bytes))
+            return RF_E_DEVICE;
+        // the sorted fans as vertex slots and index triples of one draw
+        dd.pre = 1;
+        dd.n_verts = 3 * n_f;
+        dd.n_tris = n_f;
+        dd.tris = (const uint32_t*)d.ds_idx.p;
+        if (hipMemcpyAsync(desc, &dd, sizeof(dd), hipMemcpyHostToDevice, s)) return RF_E_DEVICE;
+        B.n_verts = 3ull * n_f;
+        B.n_tris = n_f;
+        B.n_groups = (uint32_t)((n_f + WAVE_SIZE - 1) / WAVE_SIZE);
+        B.runs_cap = std::max<uint32_t>(B.runs_cap, 4 * B.n_groups + B.n_tiles + 1024);
+        B.fans = (float*)d.ds_fans.p;  // the gather reads the fans of the first launch
+        if (d.plain.prepare(B, &d.bytes) ||
+            launch_fan_gather(B, vals + cap, n_f, (uint32_t*)d.ds_idx.p, s))
+            return RF_E_DEVICE;
+        B.fans = nullptr;
+    }
 
     std::vector<FrameStats> fs;
     uint32_t err = 0;

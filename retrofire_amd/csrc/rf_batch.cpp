@@ -194,6 +194,7 @@ rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_fra
         const rf_frame& f = frames[i];
         if (!f.target || (f.n_draws && !f.draws)) return RF_E_ARG;
         rf_status st = check_ctx(&f.ctx);
+        if (!st && f.ctx.depth_sort) st = RF_E_UNSUPPORTED;  // depth sort: rf_render only
         if (st) return st;
         for (uint32_t k = 0; k < f.n_draws; ++k) {
             const rf_mesh_draw& md = f.draws[k];

@@ -44,6 +44,8 @@ struct Device {
     uint64_t bytes = 0;  // device memory held
     Scratch plain;
     DevBuf up_pos, up_attr, up_tris, up_tex, up_color, up_depth, up_desc;
+    DevBuf ds_fans, ds_keys, ds_vals, ds_tmp, ds_idx;  // depth sort: clip fans, keys, order, sort scratch
+    uint32_t ds_cap = 0;
     uint32_t plain_tile_w = 32;  // rasterizer tile width of rf_render
     uint32_t plain_extra_cap = 0, plain_runs_cap = 0;
 };
@@ -81,6 +83,16 @@ uint32_t raster_grid(const Device& d, uint32_t n_tiles);
 rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const std::vector<FrameStats*>&,
                     std::vector<FrameStats>& stats_out, uint32_t* err_out,
                     const std::function<hipError_t()>& restore);
+
+// Depth sort (rf_kernels.hip, rf_sort.hip): clip fans and their keys for the
+// single draw of `B`; the sort of (key, fan) pairs; the sorted fans written
+// as vertex slots 3i..3i+2 with index triples `idx`.
+hipError_t launch_fans(const BatchDesc& B, hipStream_t s);
+hipError_t sort_pairs_u64(void* tmp, size_t& tmp_bytes, const unsigned long long* keys_in,
+                          unsigned long long* keys_out, const uint32_t* vals_in, uint32_t* vals_out, uint32_t n,
+                          hipStream_t s);
+hipError_t launch_fan_gather(const BatchDesc& B, const uint32_t* order, uint32_t n_fans, uint32_t* idx,
+                             hipStream_t s);
 
 // Enqueues the whole pipeline for batch `B` on stream `s` (rf_kernels.hip).
 hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid);

@@ -148,6 +148,7 @@ __global__ __launch_bounds__(256) void k_vertex(BatchDesc B) {
     const uint32_t di = draw_of_vert(B.draws, B.n_draws, B.vblock_draw[slot / 256], slot);
     if (B.any_cull && !B.draw_vis[di]) return;  // a draw the bounding-box test hid: nothing reads these slots
     const DrawDesc* D = &B.draws[di];
+    if (D->pre) return;  // filled by k_fan_gather
     const uint64_t i = slot - D->vert_base;
     const float px = D->pos[3 * i + 0], py = D->pos[3 * i + 1], pz = D->pos[3 * i + 2];
     // vertex(mvp.apply(&v.pos), v.attrib): ProjMat3::apply, to_hom w = 1
@@ -1803,6 +1804,85 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
 }
 
 // ---------------------------------------------------------------------------
+// Depth sort (render.rs:180-183, 209-219), before to_screen and face culling:
+//   k_fans        one thread per triangle: assembly and clipping as k_setup
+//                 does, each resulting clip-space triangle (a clip fan, in
+//                 the order clip_tris emits them) stored with its key:
+//                 Render::depth, (a.z + b.z + c.z) / 3 (prim.rs:21-23), as a
+//                 total_cmp order key, then the triangle and fan index, so
+//                 equal depths keep submission order;
+//   radix sort of (key, fan) pairs;
+//   k_fan_gather  fan i of the sorted order becomes vertex slots 3i..3i+2
+//                 (clip position, attributes, outcode 0, screen vertex): the
+//                 main pipeline then draws them as unclipped triangles.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void put_fan(const BatchDesc& B, const CV& a, const CV& b, const CV& c, uint64_t g,
+                                        uint32_t j) {
+    const uint32_t f = atomicAdd(B.fan_count, 1u);
+    if (f >= B.fan_cap) {
+        atomicOr(B.err, ERRB_EXTRA);
+        return;
+    }
+    const float z = ((a.p[2] + b.p[2]) + c.p[2]) / 3.0f;
+    uint32_t k = (uint32_t)total_key(z) ^ 0x80000000u;  // ascending total order as unsigned
+    if (B.depth_sort == 2) k = ~k;                      // back to front
+    B.fan_keys[f] = ((unsigned long long)k << 32) | (g << 3) | j;
+    B.fan_vals[f] = f;
+    float* o = B.fans + (uint64_t)f * FAN_FLOATS;
+    const CV* v[3] = {&a, &b, &c};
+    for (int q = 0; q < 3; ++q) {
+        for (int e = 0; e < 4; ++e) o[7 * q + e] = v[q]->p[e];
+        for (int e = 0; e < 3; ++e) o[7 * q + 4 + e] = v[q]->a[e];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fans(BatchDesc B) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= B.n_tris) return;
+    const DrawDesc* D = &B.draws[0];
+    const uint32_t i0 = D->tris[3 * g], i1 = D->tris[3 * g + 1], i2 = D->tris[3 * g + 2];
+    if (i0 >= D->n_verts || i1 >= D->n_verts || i2 >= D->n_verts) {
+        atomicOr(B.err, ERRB_INDEX);
+        return;
+    }
+    CV poly[CLIP_MAX], tmp[CLIP_MAX];
+    const uint32_t ix[3] = {i0, i1, i2};
+    for (int k = 0; k < 3; ++k) {
+        const float4 p = B.cv[ix[k]], a = B.ca[ix[k]];
+        const float pp[4] = {p.x, p.y, p.z, p.w}, aa[3] = {a.x, a.y, a.z};
+        cv_make(poly[k], pp, aa);
+    }
+    const uint32_t all_out = poly[0].oc & poly[1].oc & poly[2].oc, any_out = poly[0].oc | poly[1].oc | poly[2].oc;
+    if (all_out != 0) return;  // Hidden
+    if (any_out == 0) {        // Visible
+        put_fan(B, poly[0], poly[1], poly[2], g, 0);
+        return;
+    }
+    const int n = clip_polygon(poly, 3, tmp);
+    if (n < 0) atomicOr(B.err, ERRB_CLIP);
+    for (int k = 1; k + 1 < n; ++k) put_fan(B, poly[0], poly[k], poly[k + 1], g, (uint32_t)(k - 1));
+}
+
+__global__ __launch_bounds__(256) void k_fan_gather(BatchDesc B, const uint32_t* order, uint32_t n_fans,
+                                                    uint32_t* idx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_fans) return;
+    const DrawDesc* D = &B.draws[0];
+    const float* src = B.fans + (uint64_t)order[i] * FAN_FLOATS;
+    const bool persp = kind_persp(D->kind);
+    for (int q = 0; q < 3; ++q) {
+        const uint64_t slot = 3ull * i + q;
+        CV c;
+        for (int e = 0; e < 4; ++e) c.p[e] = src[7 * q + e];
+        for (int e = 0; e < 3; ++e) c.a[e] = src[7 * q + 4 + e];
+        B.cv[slot] = make_float4(c.p[0], c.p[1], c.p[2], c.p[3]);
+        B.ca[slot] = make_float4(c.a[0], c.a[1], c.a[2], u2f(0u));  // inside: clipped already
+        to_screen(c, D->vp, persp, B.sv + 6 * slot);
+        idx[slot] = (uint32_t)slot;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_cull: one thread per draw. A draw that asks for it is skipped when its
 // mesh's bounding box is Hidden: the 8 corners (BBox::verts, scene.rs:59-66)
 // through the draw's ProjMat3 (scene.rs:81-87) and view_frustum::status
@@ -1855,6 +1935,7 @@ __global__ void k_reset(BatchDesc B) {
     if (i == 0) {
         *B.extra_count = 0;
         *B.err = 0;
+        if (B.fan_count) *B.fan_count = 0;
         for (uint32_t k = 0; k < TILE_PARTS; ++k) B.tile_counter[k] = 0;
     }
 }
@@ -1907,6 +1988,23 @@ static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raste
     mark(6);
     chk(hipGetLastError());
     return e;
+}
+hipError_t launch_fans(const BatchDesc& B, hipStream_t s) {
+    if (B.n_tris && (!B.fans || !B.fan_keys || !B.fan_vals || !B.fan_count || !B.fan_cap)) return hipErrorInvalidValue;
+    uint32_t reset_n = B.n_tiles > B.n_frames ? B.n_tiles : B.n_frames;
+    reset_n = reset_n > B.n_groups ? reset_n : B.n_groups;
+    const uint32_t n_vblocks = (uint32_t)((B.n_verts + 255) / 256);
+    reset_n = reset_n > n_vblocks ? reset_n : n_vblocks;
+    hipLaunchKernelGGL(k_reset, dim3((reset_n + 255) / 256 + 1), dim3(256), 0, s, B);
+    if (B.n_verts) hipLaunchKernelGGL(k_vertex, dim3(n_vblocks), dim3(256), 0, s, B);
+    if (B.n_tris) hipLaunchKernelGGL(k_fans, dim3((uint32_t)((B.n_tris + 255) / 256)), dim3(256), 0, s, B);
+    return hipGetLastError();
+}
+hipError_t launch_fan_gather(const BatchDesc& B, const uint32_t* order, uint32_t n_fans, uint32_t* idx,
+                             hipStream_t s) {
+    if (n_fans && (!B.fans || !order || !idx || !B.cv || !B.ca || !B.sv)) return hipErrorInvalidValue;
+    if (n_fans) hipLaunchKernelGGL(k_fan_gather, dim3((n_fans + 255) / 256), dim3(256), 0, s, B, order, n_fans, idx);
+    return hipGetLastError();
 }
 hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid) {
     return launch_impl(B, s, raster_grid, nullptr);

@@ -487,6 +487,7 @@ rf_status rf_serve_submit(rf_serve* sh, const rf_request* req) {
         return RF_E_ARG;
     Serve& s = *reinterpret_cast<Serve*>(sh);
     rf_status st = check_ctx(&req->ctx);
+    if (!st && req->ctx.depth_sort) st = RF_E_UNSUPPORTED;  // depth sort: rf_render only
     if (st) return st;
     Req q;
     q.r = *req;

@@ -93,7 +93,7 @@ struct DrawDesc {
     uint32_t attr_stride;               // floats per vertex in `attr`
     uint32_t cull;                      // 1: skipped when BBox::visibility(mvp) is Hidden (scene.rs:81-87)
     uint32_t prim;                      // PrimType; n_tris counts primitives of that type
-    uint32_t pad1;
+    uint32_t pre;                       // 1: vertex slots already filled (depth-sorted clip fans)
     uint64_t vert_base;                 // first slot in the batch vertex buffers
     uint64_t tri_base;                  // first global triangle index
     float mvp[16];                      // row-major ProjMat3 (the uniform)
@@ -174,6 +174,14 @@ struct BatchDesc {
     uint32_t* err;
     uint32_t* draw_vis;  // per draw: 0 if the bounding-box test hid it, else 1
     uint32_t any_cull;   // some draw asks for the bounding-box test
+    // depth sort (render.rs:180-183, 209-219): clip fans in clip space, their keys
+    uint32_t depth_sort;              // 0 none, 1 front to back, 2 back to front
+    uint32_t fan_cap;
+    float* fans;                      // per fan: 3 x (x, y, z, w, a0, a1, a2)
+    unsigned long long* fan_keys;     // per fan: order key << 32 | triangle << 3 | fan
+    uint32_t* fan_vals;               // per fan: its index in `fans`
+    uint32_t* fan_count;
 };
+constexpr int FAN_FLOATS = 21;
 
 }  // namespace rf

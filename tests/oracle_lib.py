@@ -17,7 +17,7 @@ _lib = None
 
 class OracleCtx(C.Structure):
     _fields_ = [("depth_test", C.c_int32), ("face_cull", C.c_int32),
-                ("color_write", C.c_int32), ("depth_write", C.c_int32)]
+                ("color_write", C.c_int32), ("depth_write", C.c_int32), ("depth_sort", C.c_int32)]
 
 
 def _p(a, t):

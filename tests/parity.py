@@ -7,6 +7,7 @@ import retrofire_amd as rf
 
 DT = {None: 0, "less": 1, "equal": 2, "greater": 3}
 FC = {None: 0, "back": 1, "front": 2}
+DS = {None: 0, "front_to_back": 1, "back_to_front": 2}
 
 
 def make_target(w, h, fmt, with_depth, seed=0, clear=None):
@@ -56,7 +57,7 @@ def run_both(kind, tris, pos, attr, mvp, vp, w, h, fmt=rf.PIX_RGBA8, with_depth=
         if odep is not None and ctx.depth_clear is not None:
             odep[...] = np.float32(1.0) / np.float32(ctx.depth_clear)
     ost = np.zeros(8, np.uint64)
-    octx = (DT[ctx.depth_test], FC[ctx.face_cull], int(ctx.color_write), int(ctx.depth_write))
+    octx = (DT[ctx.depth_test], FC[ctx.face_cull], int(ctx.color_write), int(ctx.depth_write), DS[ctx.depth_sort])
     if kind == 0:
         shd = rf.shader.Flat(flat)
     elif kind == 1:
@@ -69,7 +70,7 @@ def run_both(kind, tris, pos, attr, mvp, vp, w, h, fmt=rf.PIX_RGBA8, with_depth=
         shd = rf.shader.TexOnce(tex)
     gctx = rf.Context(color_clear=ctx.color_clear, depth_clear=ctx.depth_clear, face_cull=ctx.face_cull,
                       depth_test=ctx.depth_test, color_write=ctx.color_write, depth_write=ctx.depth_write,
-                      clear=ctx.clear)
+                      clear=ctx.clear, depth_sort=ctx.depth_sort)
     target = rf.Framebuf(gcol, gdep, fmt) if with_depth else rf.Colorbuf(gcol, fmt)
     verts = (pos, attr) if attr is not None else pos
     for i in range(calls):

@@ -9,6 +9,7 @@ import pytest
 import oracle_lib as O
 import retrofire_amd as rf
 from parity import assert_same, run_both
+from retrofire_amd import batch as RB
 from retrofire_amd import mat as M
 from retrofire_amd import scenes as S
 from test_oracle_kats import GOLDEN, hello_tri_inputs, read_ppm, textured_quad_inputs
@@ -163,10 +164,10 @@ def test_errors():
         rf.render(np.array([[0, 1, 2]]), (pos, np.zeros((3, 2), np.float32)), rf.shader.TexRepeatPot(tex),
                   M.identity(), M.viewport(0, 0, 8, 8), rf.Colorbuf(fb), rf.Context())
     assert e.value.status == -3
-    with pytest.raises(rf.RenderError) as e:
-        rf.render(np.array([[0, 1, 2]]), pos, rf.shader.Flat(), M.identity(), M.viewport(0, 0, 8, 8),
+    with pytest.raises(rf.RenderError) as e:  # index errors also under a depth sort
+        rf.render(np.array([[0, 1, 3]]), pos, rf.shader.Flat(), M.identity(), M.viewport(0, 0, 8, 8),
                   rf.Colorbuf(fb), rf.Context(depth_sort="back_to_front"))
-    assert e.value.status == -5
+    assert e.value.status == -1
     verts = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32)
     with pytest.raises(rf.RenderError) as e:  # viewport larger than the target (target.rs:173 panics)
         rf.render(np.array([[0, 1, 2]]), verts, rf.shader.Flat(), M.identity(), M.viewport(0, 0, 16, 16),
@@ -250,3 +251,49 @@ def test_extreme_attributes(kind):
     o, g = run_both(kind, tris, pos, attr, mvp, vp, w, h, tex=tex, ctx=rf.Context(clear=True, face_cull=None))
     assert_same(o, g)
     assert g[2][6] > 0
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("sort", ["front_to_back", "back_to_front"])
+@pytest.mark.parametrize("dt", [None, "less"])
+def test_depth_sort(kind, sort, dt):
+    """render() with depth_sort (render.rs:180-183, 209-219): clipped
+    triangles (near-plane fans included) drawn in depth order."""
+    rng = np.random.default_rng(50 + kind)
+    w, h = 301, 197
+    pos, attr, tris, mvp, vp = random_soup(rng, 300, w, h, kind)
+    tex = None
+    if kind >= 2:
+        tex = rng.integers(0, 256, size=(32, 64 if kind == 3 else 37, 4), dtype=np.uint8)
+    ctx = rf.Context(clear=True, depth_test=dt, face_cull=None, depth_sort=sort)
+    o, g = run_both(kind, tris, pos, attr, mvp, vp, w, h, tex=tex, flat=(9, 200, 77, 255), ctx=ctx, calls=2)
+    assert_same(o, g)
+    assert g[2][2] > 200
+
+
+def test_depth_sort_crates_and_edges():
+    sc = S.crates1080(n_crates=200)
+    for sort in ("front_to_back", "back_to_front"):
+        ctx = rf.Context(clear=True, depth_test=None, depth_sort=sort)
+        o, g = run_both(sc.kind, sc.tris, sc.pos, sc.attr, sc.mvp, sc.viewport, sc.width, sc.height, tex=sc.tex,
+                        ctx=ctx)
+        assert_same(o, g)
+    # edges keep their order under a depth sort (every Render::depth is +inf)
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(-1, 1, size=(200, 3)).astype(np.float32)
+    edges = np.arange(200, dtype=np.uint32).reshape(100, 2)
+    col = rng.uniform(0, 1, size=(200, 3)).astype(np.float32)
+    o, g = run_both(1, edges, pos, col, M.identity(), M.viewport(0, 60, 80, 0), 80, 60,
+                    ctx=rf.Context(clear=True, depth_test=None, depth_sort="back_to_front"))
+    assert_same(o, g)
+
+
+def test_depth_sort_rejected_by_batches():
+    m = RB.Mesh(np.zeros((3, 3), np.float32), np.array([[0, 1, 2]], np.uint32))
+    fb = RB.DeviceFramebuf(16, 16)
+    with pytest.raises(rf.RenderError) as e:
+        RB.Batch([(fb, rf.Context(depth_sort="front_to_back"),
+                         [RB.mesh_draw(m, rf.shader.Flat(), M.identity(), M.viewport(0, 16, 16, 0))])])
+    assert e.value.status == -5
+    fb.free()
+    m.free()

@@ -812,3 +812,46 @@ def test_span_outside_target_is_an_error():  # target.rs:173-174 panics out of b
     err, _ = O.render(0, np.array([[0, 1, 2]]), verts, None, M.identity(), M.viewport(0, 0, 16, 16),
                       fb, ctx=(0, 0, 1, 1))
     assert err == -2
+
+
+# ----------------------------------------------- depth sort (render.rs:209-219)
+
+def _flat_layers(rng, n):
+    """n screen-parallel triangles (constant view depth each, distinct),
+    overlapping in the middle of the view, with distinct flat colours drawn
+    as Gouraud attributes."""
+    pos, col = [], []
+    for k in range(n):
+        z = rng.uniform(1.0, 9.0)
+        cx, cy = rng.uniform(-0.4, 0.4, 2) * z
+        s = rng.uniform(0.3, 0.8) * z
+        pos += [(cx - s, cy - s, z), (cx + s, cy - s, z), (cx, cy + s, z)]
+        c = rng.uniform(0.05, 0.95, 3)
+        col += [c, c, c]
+    return np.array(pos, np.float32), np.array(col, np.float32), np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+
+
+def test_depth_sort_painter_equals_depth_test():
+    """Back to front with no depth test paints the nearest layer last, which
+    is what the depth test keeps without sorting; front to back with no depth
+    test keeps the farthest, as the Greater test does (ctx.rs:86-89)."""
+    rng = np.random.default_rng(21)
+    w, h = 160, 120
+    vp = M.viewport(0, h, w, 0)
+    mvp = M.perspective(np.float32(1.0), np.float32(w / h), 0.1, 20.0)
+    for _ in range(5):
+        pos, col, tris = _flat_layers(rng, 12)
+        rng.shuffle(tris)
+        imgs = {}
+        for name, ctx in (("b2f", (0, 0, 1, 1, 2)), ("less", (1, 0, 1, 1, 0)), ("f2b", (0, 0, 1, 1, 1)),
+                          ("greater", (3, 0, 1, 1, 0)), ("none", (0, 0, 1, 1, 0))):
+            fb = np.zeros((h, w, 4), np.uint8)
+            dep = np.full((h, w), np.float32(1.0 / 20.0) if name != "greater" else np.float32(1e9), np.float32)
+            if name == "less":
+                dep[...] = 0.0
+            err, _ = O.render(1, tris, pos, col, mvp, vp, fb, depth=dep, ctx=ctx)
+            assert err == 0
+            imgs[name] = fb
+        assert np.array_equal(imgs["b2f"], imgs["less"])
+        assert np.array_equal(imgs["f2b"], imgs["greater"])
+        assert not np.array_equal(imgs["b2f"], imgs["none"])  # the sort changed the order
