@@ -22,13 +22,13 @@ double now_s() {
 
 hipError_t DevBuf::ensure(size_t bytes, uint64_t* counter) {
     if (bytes <= cap) return hipSuccess;
+    size_t want = std::max(bytes, cap + cap / 2);  // grow by half at least
     if (p) {
         (void)hipFree(p);
         if (counter) *counter -= cap;
         p = nullptr;
         cap = 0;
     }
-    size_t want = std::max(bytes, cap + cap / 2);
     want = (want + 255) & ~size_t(255);
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
