@@ -926,6 +926,12 @@ __device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint
 }
 
 
+#ifndef RF_EARLY_CLEAR
+#define RF_EARLY_CLEAR 0  // workgroups that clear the tiles without runs first
+#endif
+#ifndef RF_VEC_INIT
+#define RF_VEC_INIT 1  // tile init of a cleared frame with 16-byte LDS fills
+#endif
 #ifndef RF_CLEAR_IN_RASTER
 #define RF_CLEAR_IN_RASTER 1
 #endif
@@ -1183,9 +1189,10 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
     constexpr int NPX = TW * LD;
     constexpr int NCG = TW / CELL_W;
-    __shared__ float s_dep[NPX];
-    __shared__ uint32_t s_pay[NPX];  // the pixel's colour source: payload
-    __shared__ uint16_t s_tag[NPX];  //   and tag
+    __shared__ __align__(16) float s_dep[NPX];
+    __shared__ uint32_t s_pay[NPX];                 // the pixel's colour source: payload
+    __shared__ __align__(16) uint16_t s_tag[NPX];   //   and tag
+    static_assert((NPX * 4) % 16 == 0 && (NPX * 2) % 16 == 0, "16-byte LDS fills");
     __shared__ SlotP s_slot[WAVE];
     __shared__ EdgeRec s_rec[REC_CAP];
     __shared__ uint32_t s_fq[FQ_CAP];  // g << 3 | fan
@@ -1200,6 +1207,37 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     // share triangles) meet in one XCD's L2. A wave takes TILE_GRAB tiles per
     // atomic and moves on to the next partition when its own is done.
     const uint32_t n_busy = *B.busy_count;  // tiles with runs, listed in B.tile_list
+    // The tiles without runs, EMPTY_GRAB per atomic: their fused clear is the
+    // only write. The first EARLY_CLEAR workgroups start with them, so these
+    // stores overlap the busy tiles' work; every wave takes what is left at
+    // its end.
+    const uint32_t n_empty = B.clear_in_raster ? B.n_tiles - n_busy : 0u;
+    auto clear_empties = [&]() {
+        uint32_t ecur_fi = 0xFFFFFFFFu;
+        FrameDesc E;
+        for (;;) {
+            uint32_t e0 = 0;
+            if (lane == 0) e0 = atomicAdd(&B.tile_counter[TILE_PARTS], (uint32_t)EMPTY_GRAB);
+            e0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)e0);
+            if (e0 >= n_empty) break;
+            const uint32_t ne = min((uint32_t)EMPTY_GRAB, n_empty - e0);
+            uint32_t my_tile = 0, my_fi = 0;
+            if (lane < ne) {
+                my_tile = B.tile_list[n_busy + e0 + lane];
+                my_fi = B.tile_frame[my_tile];
+            }
+            for (uint32_t j = 0; j < ne; ++j) {
+                const uint32_t tile = (uint32_t)__builtin_amdgcn_readlane((int)my_tile, (int)j);
+                const uint32_t fi = (uint32_t)__builtin_amdgcn_readlane((int)my_fi, (int)j);
+                if (fi != ecur_fi) {
+                    E = B.frames[fi];
+                    ecur_fi = fi;
+                }
+                clear_tile<TW>(E, tile, lane);
+            }
+        }
+    };
+    if (blockIdx.x < (uint32_t)RF_EARLY_CLEAR) clear_empties();
     uint32_t part = blockIdx.x % TILE_PARTS, parts_left = TILE_PARTS;
     uint32_t tcur = 0, tend = 0;
     bool tail_near = false;
@@ -1251,7 +1289,16 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
 
         // tile init (WAVE / TILE_H lanes per row, each a share of the columns):
         // depth from the fused clear or the target; no fragment yet
-        {
+        if (RF_VEC_INIT && (F.clear_depth || !has_depth)) {
+            // constant fills of the whole arrays (padding and any part beyond
+            // the target included, never written back), 16 bytes a store
+            const float cd = F.clear_depth_value;
+            if (has_depth)
+                for (uint32_t i = lane; i < (uint32_t)NPX / 4; i += WAVE)
+                    reinterpret_cast<float4*>(s_dep)[i] = make_float4(cd, cd, cd, cd);
+            for (uint32_t i = lane; i < (uint32_t)NPX / 8; i += WAVE)
+                reinterpret_cast<uint4*>(s_tag)[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        } else {
             constexpr uint32_t LPR = WAVE / TILE_H;  // lanes per row
             const uint32_t r = lane % TILE_H, part = lane / TILE_H;
             if (r < nrows) {
@@ -1774,33 +1821,9 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
         __syncthreads();
     }
 
-    // Then the tiles without runs, EMPTY_GRAB per atomic (waves that finish
-    // the busy tiles early fill the tail with them): their fused clear is the
-    // only write.
-    const uint32_t n_empty = B.clear_in_raster ? B.n_tiles - n_busy : 0u;
-    uint32_t ecur_fi = 0xFFFFFFFFu;
-    FrameDesc E;
-    for (;;) {
-        uint32_t e0 = 0;
-        if (lane == 0) e0 = atomicAdd(&B.tile_counter[TILE_PARTS], (uint32_t)EMPTY_GRAB);
-        e0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)e0);
-        if (e0 >= n_empty) break;
-        const uint32_t ne = min((uint32_t)EMPTY_GRAB, n_empty - e0);
-        uint32_t my_tile = 0, my_fi = 0;
-        if (lane < ne) {
-            my_tile = B.tile_list[n_busy + e0 + lane];
-            my_fi = B.tile_frame[my_tile];
-        }
-        for (uint32_t j = 0; j < ne; ++j) {
-            const uint32_t tile = (uint32_t)__builtin_amdgcn_readlane((int)my_tile, (int)j);
-            const uint32_t fi = (uint32_t)__builtin_amdgcn_readlane((int)my_fi, (int)j);
-            if (fi != ecur_fi) {
-                E = B.frames[fi];
-                ecur_fi = fi;
-            }
-            clear_tile<TW>(E, tile, lane);
-        }
-    }
+    // Then the tiles without runs (waves that finish the busy tiles early
+    // fill the tail with them).
+    clear_empties();
 }
 
 // ---------------------------------------------------------------------------
