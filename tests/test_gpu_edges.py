@@ -163,3 +163,13 @@ def test_edge_errors():
         rf.render(np.array([[0, 1]]), pos, rf.shader.Flat(), M.identity(), M.viewport(0, 8, 8, 0),
                   rf.Colorbuf(fb), rf.Context(face_cull=None))
     assert e.value.status == -2
+
+
+@pytest.mark.parametrize("fmt", [rf.PIX_RGB8, rf.PIX_U32_XRGB, rf.PIX_U32_BGRA, rf.PIX_RGBA4444, rf.PIX_RGB565])
+def test_edges_pixel_formats(fmt):
+    rng = np.random.default_rng(31)
+    w, h = 157, 93
+    pos, attr, edges, mvp, vp = edge_soup(rng, 250, w, h, 1)
+    for ctx, with_depth in ((rf.Context(clear=True, face_cull=None), True), (rf.Context(clear=False), False)):
+        o, g = run_both(1, edges, pos, attr, mvp, vp, w, h, fmt=fmt, with_depth=with_depth, ctx=ctx)
+        assert_same(o, g)

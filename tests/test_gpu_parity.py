@@ -297,3 +297,16 @@ def test_depth_sort_rejected_by_batches():
     assert e.value.status == -5
     fb.free()
     m.free()
+
+
+@pytest.mark.parametrize("fmt", [rf.PIX_RGB8, rf.PIX_U32_ARGB, rf.PIX_RGB565])
+def test_depth_sort_pixel_formats(fmt):
+    rng = np.random.default_rng(77)
+    w, h = 211, 149
+    pos, attr, tris, mvp, vp = random_soup(rng, 200, w, h, 1)
+    ctx = rf.Context(clear=True, depth_test=None, face_cull=None, depth_sort="back_to_front")
+    o, g = run_both(1, tris, pos, attr, mvp, vp, w, h, fmt=fmt, ctx=ctx)
+    assert_same(o, g)
+    o, g = run_both(1, tris, pos, attr, mvp, vp, w, h, fmt=fmt, with_depth=False,
+                    ctx=rf.Context(clear=False, depth_sort="front_to_back"))
+    assert_same(o, g)
