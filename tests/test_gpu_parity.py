@@ -310,3 +310,14 @@ def test_depth_sort_pixel_formats(fmt):
     o, g = run_both(1, tris, pos, attr, mvp, vp, w, h, fmt=fmt, with_depth=False,
                     ctx=rf.Context(clear=False, depth_sort="front_to_back"))
     assert_same(o, g)
+
+
+@pytest.mark.parametrize("w,h", [(16384, 256), (200, 16384)])
+def test_largest_target_sides(w, h):
+    """Targets at the largest side (16,384 pixels): the widest and tallest
+    tile grids, record x packed in 16 bits."""
+    sc = S.sphere640()
+    vp = M.viewport(0, 0, w, h)
+    o, g = run_both(sc.kind, sc.tris, sc.pos, sc.attr, sc.mvp, vp, w, h, ctx=rf.Context(clear=True))
+    assert_same(o, g)
+    assert g[2][6] > 1000
