@@ -120,6 +120,8 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
     if ((e = misc.ensure(256, counter))) return e;
     if ((e = stats.ensure(std::max<uint64_t>(B.n_frames, 1) * sizeof(FrameStats), counter))) return e;
     if ((e = draw_vis.ensure(std::max<uint64_t>(B.n_draws, 1) * sizeof(uint32_t), counter))) return e;
+    if ((e = prims_part.ensure(std::max<uint64_t>(B.n_frames, 1) * PRIM_PARTS * PRIM_PART_STRIDE * 8, counter)))
+        return e;
     B.cv = (float4*)cv.p;
     B.ca = (float4*)ca.p;
     B.sv = (float*)sv.p;
@@ -142,12 +144,13 @@ hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
     B.err = (uint32_t*)misc.p + 2;
     B.stats = (FrameStats*)stats.p;
     B.draw_vis = (uint32_t*)draw_vis.p;
+    B.prims_part = (unsigned long long*)prims_part.p;
     return hipSuccess;
 }
 
 void Scratch::release(uint64_t* counter) {
     for (DevBuf* b : {&cv, &ca, &sv, &hdr, &scr, &extra, &rect, &run_count, &run_off, &run_cursor,
-                      &scan_blk, &tile_frame, &tile_list, &group_draw, &vblock_draw, &runs, &misc, &stats, &draw_vis})
+                      &scan_blk, &tile_frame, &tile_list, &group_draw, &vblock_draw, &runs, &misc, &stats, &draw_vis, &prims_part})
         b->release(counter);
 }
 

@@ -32,7 +32,7 @@ struct DevBuf {
 // Per-pipeline scratch buffers.
 struct Scratch {
     DevBuf cv, ca, sv, hdr, scr, extra, rect, run_count, run_off, run_cursor, scan_blk, tile_frame, tile_list, group_draw, vblock_draw, runs, misc,
-        stats, draw_vis;
+        stats, draw_vis, prims_part;
     hipError_t prepare(BatchDesc& B, uint64_t* counter);
     void release(uint64_t* counter);
 };

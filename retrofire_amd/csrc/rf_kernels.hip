@@ -584,20 +584,18 @@ __global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
         B.rect[2 * g] = has_rect ? rect[0] : 0xFFFFFFFFu;
         B.rect[2 * g + 1] = rect[1];
     }
-    // prims.o / verts.o (render.rs:195-196): one atomic per wave when the
-    // wave's triangles all belong to one frame, else one per lane
+    // prims.o (render.rs:195; verts.o is 3 per primitive, derived in
+    // k_scan_top): one atomic per wave when the wave's triangles all belong to
+    // one frame, else one per lane, into the group's partial counter
     {
         const uint32_t fmin = wave_min_u32(nf > 0 ? frame : 0xFFFFFFFFu);
         const uint32_t fmax = ~wave_min_u32(nf > 0 ? ~frame : 0xFFFFFFFFu);
+        const uint32_t part = (group % PRIM_PARTS) * PRIM_PART_STRIDE;
         if (fmin == fmax) {
             const unsigned long long tot = wave_sum_u64(nf);
-            if (lane == 0 && tot) {
-                atomicAdd(&B.stats[fmin].prims_o, tot);
-                atomicAdd(&B.stats[fmin].verts_o, 3ull * tot);
-            }
+            if (lane == 0 && tot) atomicAdd(&B.prims_part[(uint64_t)fmin * PRIM_PARTS * PRIM_PART_STRIDE + part], tot);
         } else if (nf > 0) {
-            atomicAdd(&B.stats[frame].prims_o, (unsigned long long)nf);
-            atomicAdd(&B.stats[frame].verts_o, 3ull * nf);
+            atomicAdd(&B.prims_part[(uint64_t)frame * PRIM_PARTS * PRIM_PART_STRIDE + part], (unsigned long long)nf);
         }
     }
     // per-tile run counts
@@ -684,6 +682,14 @@ __global__ __launch_bounds__(1024) void k_scan_top(BatchDesc B) {
         if (tid < 2)
             for (uint32_t k = 0; k < 16; ++k) carry[tid] += wave_tot[tid][k];
         __syncthreads();
+    }
+    // prims.o of every frame from its partial counters; verts.o is 3 per
+    // primitive (render.rs:196)
+    for (uint32_t f = tid; f < B.n_frames; f += blockDim.x) {
+        unsigned long long p = 0;
+        for (uint32_t k = 0; k < PRIM_PARTS; ++k) p += B.prims_part[((uint64_t)f * PRIM_PARTS + k) * PRIM_PART_STRIDE];
+        B.stats[f].prims_o = p;
+        B.stats[f].verts_o = 3ull * p;
     }
     if (tid == 0) {
         B.run_off[B.n_tiles] = carry[0];
@@ -1955,6 +1961,7 @@ __global__ void k_reset(BatchDesc B) {
     // the draw of each group's first triangle and of each block's first vertex
     if (i < B.n_groups) B.group_draw[i] = find_draw_tri(B.draws, B.n_draws, (uint64_t)i * WAVE);
     if ((uint64_t)i * 256 < B.n_verts) B.vblock_draw[i] = find_draw_vert(B.draws, B.n_draws, (uint64_t)i * 256);
+    if (i < B.n_frames * PRIM_PARTS) B.prims_part[(uint64_t)i * PRIM_PART_STRIDE] = 0;
     if (i < B.n_frames) {
         B.stats[i].prims_o = 0;
         B.stats[i].verts_o = 0;
@@ -1980,7 +1987,7 @@ static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raste
     // the rasterizer's waves clear the tiles without runs at their end (faster,
     // measured, than k_bin's extra blocks doing it beside the binning)
     B.clear_in_raster = RF_CLEAR_IN_RASTER;
-    uint32_t reset_n = B.n_tiles > B.n_frames ? B.n_tiles : B.n_frames;
+    uint32_t reset_n = B.n_tiles > B.n_frames * PRIM_PARTS ? B.n_tiles : B.n_frames * PRIM_PARTS;
     reset_n = reset_n > B.n_groups ? reset_n : B.n_groups;
     const uint32_t n_vblocks = (uint32_t)((B.n_verts + 255) / 256);
     reset_n = reset_n > n_vblocks ? reset_n : n_vblocks;
@@ -2023,7 +2030,7 @@ static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raste
 }
 hipError_t launch_fans(const BatchDesc& B, hipStream_t s) {
     if (B.n_tris && (!B.fans || !B.fan_keys || !B.fan_vals || !B.fan_count || !B.fan_cap)) return hipErrorInvalidValue;
-    uint32_t reset_n = B.n_tiles > B.n_frames ? B.n_tiles : B.n_frames;
+    uint32_t reset_n = B.n_tiles > B.n_frames * PRIM_PARTS ? B.n_tiles : B.n_frames * PRIM_PARTS;
     reset_n = reset_n > B.n_groups ? reset_n : B.n_groups;
     const uint32_t n_vblocks = (uint32_t)((B.n_verts + 255) / 256);
     reset_n = reset_n > n_vblocks ? reset_n : n_vblocks;

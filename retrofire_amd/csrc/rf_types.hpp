@@ -117,6 +117,12 @@ struct FrameDesc {
 };
 
 // Per-frame statistics (stats.rs:16-40), accumulated by the kernels.
+// prims.o is counted in this many partial counters per frame, each on its
+// own 128-byte line: atomics on one address serialise, and a frame of 3.2M
+// small triangles has 50,000 setup waves adding to it.
+constexpr uint32_t PRIM_PARTS = 16;
+constexpr uint32_t PRIM_PART_STRIDE = 16;  // u64 words between partial counters
+
 struct FrameStats {
     unsigned long long prims_o, verts_o, frags_i, frags_o;
     unsigned long long hidden_draws, hidden_prims, hidden_verts;  // draws skipped by the bounding-box test
@@ -172,6 +178,7 @@ struct BatchDesc {
     uint32_t runs_cap;
     uint32_t* tile_counter;  // persistent raster work counters, one per tile partition (TILE_PARTS)
     uint32_t* err;
+    unsigned long long* prims_part;  // per frame: PRIM_PARTS partial prims.o counters
     uint32_t* draw_vis;  // per draw: 0 if the bounding-box test hid it, else 1
     uint32_t any_cull;   // some draw asks for the bounding-box test
     // depth sort (render.rs:180-183, 209-219): clip fans in clip space, their keys
