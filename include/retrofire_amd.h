@@ -159,7 +159,7 @@ rf_status rf_device_close(rf_device* dev);
 const char* rf_status_str(rf_status s);
 /* Bytes of device memory the library holds for scratch and residency. */
 uint64_t rf_device_bytes(const rf_device* dev);
-/* Rasterizer tile width (32 or 64 pixel columns; 0 = the default, 32) used by
+/* Rasterizer tile width (16, 32 or 64 pixel columns; 0 = the default, 32) used by
  * rf_render on this device. No counterpart in the reference (a tuning knob of
  * this implementation); results are identical for every width. */
 rf_status rf_device_set_tile_width(rf_device* dev, uint32_t tile_w);
@@ -230,7 +230,7 @@ typedef struct rf_frame {
 
 typedef struct rf_batch rf_batch;
 /* Plans a batch: sizes scratch from one eager run, picks the tile width
- * (tile_w = 0: autotune; 32 or 64 to force) and captures the launch sequence
+ * (tile_w = 0: autotune; 16, 32 or 64 to force) and captures the launch sequence
  * into a HIP graph. The frames' descriptors are copied; the meshes, textures
  * and framebuffers must stay alive while the batch exists. */
 rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_frames,

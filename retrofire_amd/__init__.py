@@ -176,7 +176,7 @@ class Device:
         return int(_ffi.lib().rf_device_bytes(self.h))
 
     def set_tile_width(self, tile_w: int = 0):
-        """Rasterizer tile width of render() on this device (32, 64; 0 = default)."""
+        """Rasterizer tile width of render() on this device (16, 32, 64; 0 = default)."""
         check(_ffi.lib().rf_device_set_tile_width(self.h, tile_w))
 
 

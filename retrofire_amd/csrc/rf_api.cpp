@@ -267,7 +267,7 @@ uint64_t rf_device_bytes(const rf_device* dev) {
 uint32_t rf_tile_height(void) { return TILE_ROWS; }
 
 rf_status rf_device_set_tile_width(rf_device* dev, uint32_t tile_w) {
-    if (!dev || (tile_w != 0 && tile_w != 32 && tile_w != 64)) return RF_E_ARG;
+    if (!dev || (tile_w != 0 && tile_w != 16 && tile_w != 32 && tile_w != 64)) return RF_E_ARG;
     reinterpret_cast<Device*>(dev)->plain_tile_w = tile_w ? tile_w : 32;
     return RF_OK;
 }

@@ -188,7 +188,7 @@ extern "C" {
 rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_frames, uint32_t tile_w,
                           rf_batch** out) {
     if (!dev || !out || !frames || n_frames == 0) return RF_E_ARG;
-    if (tile_w != 0 && tile_w != 32 && tile_w != 64) return RF_E_ARG;
+    if (tile_w != 0 && tile_w != 16 && tile_w != 32 && tile_w != 64) return RF_E_ARG;
     Device& d = *reinterpret_cast<Device*>(dev);
     for (uint32_t i = 0; i < n_frames; ++i) {
         const rf_frame& f = frames[i];
@@ -225,7 +225,7 @@ rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_fra
         return RF_E_DEVICE;
     }
     // Tile width: forced, or the faster of the candidates on this batch.
-    std::vector<uint32_t> cands = tile_w ? std::vector<uint32_t>{tile_w} : std::vector<uint32_t>{32, 64};
+    std::vector<uint32_t> cands = tile_w ? std::vector<uint32_t>{tile_w} : std::vector<uint32_t>{16, 32, 64};
     float best = 1e30f;
     uint32_t best_w = cands[0];
     for (uint32_t w : cands) {

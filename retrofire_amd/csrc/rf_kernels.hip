@@ -1009,7 +1009,9 @@ __device__ __forceinline__ void clear_empty_tiles(const BatchDesc& B, uint32_t w
             F = B.frames[fi];
             cur_fi = fi;
         }
-        if (B.tile_w == 32)
+        if (B.tile_w == 16)
+            clear_tile<16>(F, tile, lane);
+        else if (B.tile_w == 32)
             clear_tile<32>(F, tile, lane);
         else
             clear_tile<64>(F, tile, lane);
@@ -2020,7 +2022,9 @@ static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raste
     mark(4);
     hipLaunchKernelGGL(k_binsort, dim3((B.n_tiles + SORT_TPW - 1) / SORT_TPW), dim3(WAVE), 0, s, B);
     mark(5);
-    if (B.tile_w == 32)
+    if (B.tile_w == 16)
+        hipLaunchKernelGGL(k_raster<16>, dim3(raster_grid), dim3(64), 0, s, B);
+    else if (B.tile_w == 32)
         hipLaunchKernelGGL(k_raster<32>, dim3(raster_grid), dim3(64), 0, s, B);
     else
         hipLaunchKernelGGL(k_raster<64>, dim3(raster_grid), dim3(64), 0, s, B);

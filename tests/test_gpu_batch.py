@@ -74,7 +74,7 @@ def test_batched_uneven_frames_match_oracle(world):
                               texture=tex),
                  RB.mesh_draw(icos, rf.shader.Gouraud(), f["mvp_i"], f["vp"], n_tris=f["ni"])]
         spec.append((fb, rf.Context(clear=True), draws))
-    for tw in (0, 32, 64):
+    for tw in (0, 16, 32, 64):
         b = RB.Batch(spec, tile_w=tw)
         for rep in range(3):  # graph replays give the same result every time
             b.run(sync=True)

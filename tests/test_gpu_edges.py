@@ -15,7 +15,7 @@ from retrofire_amd import scenes as S
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=[32, 64], ids=["tw32", "tw64"])
+@pytest.fixture(autouse=True, params=[16, 32, 64], ids=["tw16", "tw32", "tw64"])
 def tile_width(request):
     dev = rf.default_device()
     dev.set_tile_width(request.param)
@@ -106,7 +106,7 @@ def test_wireframe_over_triangles_one_launch():
     shd = rf.shader.Flat(sc.flat) if sc.kind == 0 else rf.shader.Gouraud()
     fb = RB.DeviceFramebuf(w, h, 0, True)
     draws = [RB.mesh_draw(mesh, shd, sc.mvp, sc.viewport), RB.mesh_draw(wire, rf.shader.Gouraud(), sc.mvp, sc.viewport)]
-    for tw in (32, 64):
+    for tw in (16, 32, 64):
         b = RB.Batch([(fb, rf.Context(clear=True), draws)], tile_w=tw)
         b.run(sync=True)
         s = b.wait()[0]

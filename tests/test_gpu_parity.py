@@ -17,7 +17,7 @@ from test_oracle_kats import GOLDEN, hello_tri_inputs, read_ppm, textured_quad_i
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=[32, 64], ids=["tw32", "tw64"])
+@pytest.fixture(autouse=True, params=[16, 32, 64], ids=["tw16", "tw32", "tw64"])
 def tile_width(request):
     """Every parity case runs with both rasterizer tile widths."""
     dev = rf.default_device()

@@ -14,7 +14,7 @@ from test_gpu_parity import random_soup
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=[32, 64], ids=["tw32", "tw64"])
+@pytest.fixture(autouse=True, params=[16, 32, 64], ids=["tw16", "tw32", "tw64"])
 def tile_width(request):
     dev = rf.default_device()
     dev.set_tile_width(request.param)
