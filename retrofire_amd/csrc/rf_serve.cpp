@@ -230,7 +230,10 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     std::vector<FrameDesc> fds;
     uint64_t vbase = 0, tbase = 0;
     uint32_t tile_base = 0;
-    const uint32_t tile_w = 64;
+#ifndef RF_SERVE_TILE_W
+#define RF_SERVE_TILE_W 32
+#endif
+    const uint32_t tile_w = RF_SERVE_TILE_W;  // rasterizer tile width of served launches
     uint32_t any_cull = 0;
     for (size_t i = 0; i < l.reqs.size(); ++i) {
         Req& q = l.reqs[i];
