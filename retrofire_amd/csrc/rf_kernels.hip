@@ -941,6 +941,9 @@ __device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint
 }
 
 
+#ifndef RF_CLEAR_ROWS
+#define RF_CLEAR_ROWS 1  // clear runs of empty tiles row segment by row segment
+#endif
 #ifndef RF_EARLY_CLEAR
 #define RF_EARLY_CLEAR 0  // workgroups that clear the tiles without runs first
 #endif
@@ -951,7 +954,7 @@ __device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint
 #define RF_CLEAR_IN_RASTER 1
 #endif
 #ifndef RF_EMPTY_GRAB
-#define RF_EMPTY_GRAB 16
+#define RF_EMPTY_GRAB 32
 #endif
 constexpr int EMPTY_GRAB = RF_EMPTY_GRAB;  // tiles without runs a wave takes at once
 
@@ -1242,6 +1245,44 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
             if (lane < ne) {
                 my_tile = B.tile_list[n_busy + e0 + lane];
                 my_fi = B.tile_frame[my_tile];
+            }
+            if (RF_CLEAR_ROWS) {
+                // ne consecutive full tiles of one tile row of one frame (the
+                // common case: the list is in tile order): clear the row
+                // segment they make, each store instruction 1 KB of one pixel row
+                const uint32_t t0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)my_tile);
+                const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)my_fi);
+                const bool run = __ballot(lane < ne && (my_tile != t0 + lane || my_fi != f0)) == 0ull;
+                if (run) {
+                    if (f0 != ecur_fi) {
+                        E = B.frames[f0];
+                        ecur_fi = f0;
+                    }
+                    const uint32_t lt = t0 - E.tile_base, tx = lt % E.ntx, ty = lt / E.ntx;
+                    const uint32_t x0 = tx * TW, y0 = ty * TILE_H;
+                    const bool has_depth = E.depth != nullptr;
+                    const bool ok = tx + ne <= E.ntx && x0 + ne * TW <= E.width && y0 + TILE_H <= E.height &&
+                                    pixfmt_bytes(E.fmt) == 4u && ((uintptr_t)E.color & 15u) == 0 &&
+                                    (E.color_stride & 15u) == 0 &&
+                                    (!has_depth || (((uintptr_t)E.depth & 15u) == 0 && (E.depth_stride & 3u) == 0));
+                    if (ok) {
+                        if (E.clear_color || (has_depth && E.clear_depth)) {
+                            const uint32_t qpr = ne * (TW / 4);  // 16-byte quads per pixel row
+                            const uint32_t cp = E.clear_pixel;
+                            const float cz = E.clear_depth_value;
+                            for (uint32_t q = lane; q < TILE_H * qpr; q += WAVE) {
+                                const uint32_t r = q / qpr, c = (q - r * qpr) * 4;
+                                if (E.clear_color)
+                                    *(uint4*)(E.color + (uint64_t)(y0 + r) * E.color_stride + 4ull * (x0 + c)) =
+                                        make_uint4(cp, cp, cp, cp);
+                                if (has_depth && E.clear_depth)
+                                    *(float4*)(E.depth + (uint64_t)(y0 + r) * E.depth_stride + x0 + c) =
+                                        make_float4(cz, cz, cz, cz);
+                            }
+                        }
+                        continue;
+                    }
+                }
             }
             for (uint32_t j = 0; j < ne; ++j) {
                 const uint32_t tile = (uint32_t)__builtin_amdgcn_readlane((int)my_tile, (int)j);
