@@ -153,7 +153,9 @@ typedef struct rf_texture rf_texture;
 typedef struct rf_framebuf rf_framebuf;
 typedef struct rf_serve rf_serve;
 
-/* ---- device ---------------------------------------------------------- */
+/* ---- device ----------------------------------------------------------
+ * No counterpart in the reference (its render() needs no device): one handle
+ * per GPU, holding its stream and the scratch of rf_render. */
 rf_status rf_device_open(int ordinal, rf_device** out);
 rf_status rf_device_close(rf_device* dev);
 const char* rf_status_str(rf_status s);
@@ -204,7 +206,10 @@ rf_status rf_framebuf_read(rf_device* dev, const rf_framebuf* fb, void* color, f
 
 /* ---- batched path -------------------------------------------------------
  * Many frames of uneven resolution, each an ordered list of draws over
- * resident meshes, go through one pipeline launch. */
+ * resident meshes, go through one pipeline launch. Each draw is one render()
+ * call (render.rs:134-207) as the frame loops of Batch::render
+ * (batch.rs:127-146) and Camera::render (cam.rs:229-254) make them; the
+ * batch itself has no counterpart in the reference. */
 typedef struct rf_mesh_draw {
     const rf_mesh* mesh;
     uint32_t kind;
@@ -252,7 +257,9 @@ rf_status rf_batch_profile(rf_batch* b, int iters, float* ms);
 /* ---- serving ------------------------------------------------------------
  * A request loop that keeps meshes and textures resident, admits frames by a
  * device-memory budget and batches whatever is admitted while earlier work is
- * still on the GPU (continuous batching over `n_streams` streams). */
+ * still on the GPU (continuous batching over `n_streams` streams). No
+ * counterpart in the reference; every request is a frame of render() calls
+ * cleared first, as the frontends' loops do (front/src/lib.rs:103-120). */
 typedef struct rf_request {
     uint64_t id;
     const rf_mesh_draw* draws;   /* copied at submit */
