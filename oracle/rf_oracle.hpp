@@ -17,7 +17,16 @@
 //   - raster.rs:326-368 shared-edge grid, raster.rs:371-401 gradient grid,
 //     raster.rs:404-436 fragment KATs
 //   - clip.rs:425-664 exact clip outputs, clip.rs:667-719 exhaustive histogram
-//   - tex.rs:381-418 sampler KATs, vary.rs:164-191 stepping KATs
+//   - tex.rs:381-418 sampler KATs (SamplerOnce included), vary.rs:164-191
+//     stepping KATs
+//   - pixfmt.rs:143-213 pixel format KATs (u32, byte and u16 forms)
+//   - scene.rs:144-153 BBox extend; closed forms and brute force for
+//     BBox::visibility
+//   - clip.rs:455-486 edge KATs through the edge-list clipper; brute force
+//     for edge clipping and raster::line geometry; the first-end varying rule
+//   - depth sort: back to front without depth test equals the Less test
+//     (screen-parallel layers); the order of equal depths is parity unpinned
+//     (the reference leaves it to Rust's sort_unstable_by)
 //   - rand.rs doctests (Xorshift64, Uniform<f32>)
 //   - brute-force and closed-form checks in tests/test_oracle_kats.py
 // Parity of bit patterns on arbitrary triangles beyond these pins rests on
