@@ -721,7 +721,8 @@ static_assert(SORT_TPW >= 1 && SORT_TPW <= 64, "one lane per tile");
 // Sorts the runs held one per lane (lanes >= n hold none) by group: a bitonic
 // network over the 64 lanes; lane i ends with the i-th run. Groups are
 // distinct within a tile.
-__device__ __forceinline__ void wave_sort_runs(uint32_t n, uint32_t& group, unsigned long long& mask) {
+__device__ __forceinline__ void wave_sort_runs(uint32_t n, uint32_t& group, unsigned long long& mask,
+                                               uint32_t& multi) {
     const uint32_t lane = lane_id();
     const bool in = lane < n;
     const uint32_t g = in ? group : 0xFFFFFFFFu;
@@ -736,6 +737,7 @@ __device__ __forceinline__ void wave_sort_runs(uint32_t n, uint32_t& group, unsi
         }
     }
     mask = __shfl(mask, (uint32_t)(kk & 63u), WAVE);
+    multi = __shfl(multi, (uint32_t)(kk & 63u), WAVE);
     group = (uint32_t)(kk >> 6);
 }
 
@@ -770,7 +772,9 @@ __global__ __launch_bounds__(64) void k_binsort(BatchDesc B) {
             uint32_t p2 = 1;
             while (p2 < nn) p2 <<= 1;
             for (uint32_t i = lane; i < nn; i += WAVE) {
-                key[i] = ((unsigned long long)runs[i].group << 32) | i;
+                // groups are distinct in a tile, so the flag and index below
+                // the group never decide the order
+                key[i] = ((unsigned long long)runs[i].group << 32) | ((unsigned long long)runs[i].multi << 31) | i;
                 msk[i] = runs[i].mask;
             }
             __syncthreads();
@@ -796,8 +800,8 @@ __global__ __launch_bounds__(64) void k_binsort(BatchDesc B) {
             for (uint32_t i = lane; i < nn; i += WAVE) {
                 Run w;
                 w.group = (uint32_t)(key[i] >> 32);
-                w.pad = 0;
-                w.mask = msk[(uint32_t)key[i]];
+                w.multi = (uint32_t)(key[i] >> 31) & 1u;
+                w.mask = msk[(uint32_t)key[i] & 0x7FFFFFFFu];
                 runs[i] = w;
             }
             __syncthreads();
@@ -941,6 +945,9 @@ __device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint
 }
 
 
+#ifndef RF_RUN_MULTI
+#define RF_RUN_MULTI 1  // runs flag groups with several fans per triangle
+#endif
 #ifndef RF_CLEAR_ROWS
 #define RF_CLEAR_ROWS 1  // clear runs of empty tiles row segment by row segment
 #endif
@@ -1047,6 +1054,8 @@ __global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
         }
     }
     const uint32_t cap = B.runs_cap;
+    // whether a triangle of the group has more than one clip fan
+    const uint32_t multi = __ballot(valid && (B.hdr[g] & 7u) > 1u) ? 1u : 0u;
     // The tiles of the group in increasing order (as in for_each_tile); lane k
     // collects the k-th (tile, mask), and every 64 of them take their run slots
     // with one round of atomics.
@@ -1060,7 +1069,7 @@ __global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
             if (pos < cap) {
                 Run r;
                 r.group = group;
-                r.pad = 0;
+                r.multi = multi;
                 r.mask = my_mask;
                 B.runs[pos] = r;
             }
@@ -1377,15 +1386,16 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
 
         // the tile's runs in group order: lists of up to 64 are loaded one per
         // lane and sorted here; longer ones were sorted by k_binsort
-        uint32_t lrun_g = 0;
+        uint32_t lrun_g = 0, lrun_x = 0;
         unsigned long long lrun_m = 0;
         if (nruns <= (uint32_t)WAVE) {
             if (lane < nruns) {
                 const Run r = B.runs[off + lane];
                 lrun_g = r.group;
                 lrun_m = r.mask;
+                lrun_x = r.multi;
             }
-            wave_sort_runs(nruns, lrun_g, lrun_m);
+            wave_sort_runs(nruns, lrun_g, lrun_m, lrun_x);
         }
         // Batches of up to 64 fans follow one another without a pause: once a
         // batch has handed out its last record, the next one is set up while
@@ -1408,12 +1418,15 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                     if (nruns <= (uint32_t)WAVE) {
                         run.group = __shfl(lrun_g, ri, WAVE);
                         run.mask = __shfl(lrun_m, ri, WAVE);
+                        run.multi = __shfl(lrun_x, ri, WAVE);
                     } else {
                         run = B.runs[off + ri];
                     }
                     const uint64_t g = (uint64_t)run.group * WAVE + lane;
                     const bool has = (run.mask >> lane) & 1ull;
-                    const uint32_t nf = has ? (B.hdr[g] & 7u) : 0u;
+                    // a triangle in the mask has at least one fan; only groups
+                    // with clipped triangles need their headers
+                    const uint32_t nf = has ? (RF_RUN_MULTI && !run.multi ? 1u : (B.hdr[g] & 7u)) : 0u;
                     uint32_t tot;
                     const uint32_t pre = wave_excl_scan(nf, &tot);
                     for (uint32_t j = 0; j < nf; ++j) s_fq[(fq_tail + pre + j) % FQ_CAP] = ((uint32_t)g << 3) | j;

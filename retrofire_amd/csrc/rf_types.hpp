@@ -140,7 +140,7 @@ struct ScrTri {
 // indices) that touch one tile, as a bit mask.
 struct Run {
     uint32_t group;
-    uint32_t pad;
+    uint32_t multi;  // 1: some triangle of the group has several clip fans (read its header)
     unsigned long long mask;
 };
 
