@@ -226,3 +226,29 @@ def test_object_culling_matches_caller_side_culling():
         fb.free()
     cube.free()
     tex.free()
+
+
+def test_argument_checks():
+    """Invalid tile widths, culling flags and shader kinds are rejected before
+    any work is queued."""
+    from retrofire_amd import _ffi
+    import ctypes as C
+    dev = rf.default_device()
+    assert _ffi.lib().rf_device_set_tile_width(dev.h, 48) == -3
+    m = RB.Mesh(np.zeros((3, 3), np.float32), np.array([[0, 1, 2]], np.uint32))
+    fb = RB.DeviceFramebuf(16, 16)
+    d = RB.mesh_draw(m, rf.shader.Flat(), M.identity(), M.viewport(0, 16, 16, 0))
+    with pytest.raises(rf.RenderError) as e:
+        RB.Batch([(fb, rf.Context(), [d])], tile_w=48)
+    assert e.value.status == -3
+    d.cull = 2
+    with pytest.raises(rf.RenderError) as e:
+        RB.Batch([(fb, rf.Context(), [d])])
+    assert e.value.status == -3
+    d.cull = 0
+    d.kind = 9
+    with pytest.raises(rf.RenderError) as e:
+        RB.Batch([(fb, rf.Context(), [d])])
+    assert e.value.status == -3
+    fb.free()
+    m.free()
