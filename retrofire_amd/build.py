@@ -12,7 +12,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librfamd.so")
-SOURCES = ["rf_kernels.hip", "rf_sort.hip", "rf_api.cpp", "rf_batch.cpp", "rf_serve.cpp"]
+# (object name, source, extra flags). rf_kernels.hip builds twice: the pipeline
+# kernels, and k_raster alone under the max-ilp scheduler (shorter dependent
+# chains in the rasterizer; the same scheduler makes k_setup slower).
+SOURCES = [
+    ("rf_kernels", "rf_kernels.hip", []),
+    ("rf_raster", "rf_kernels.hip", ["-DRF_RASTER_TU", "-mllvm", "-amdgpu-sched-strategy=max-ilp"]),
+    ("rf_sort", "rf_sort.hip", []),
+    ("rf_api", "rf_api.cpp", []),
+    ("rf_batch", "rf_batch.cpp", []),
+    ("rf_serve", "rf_serve.cpp", []),
+]
 HEADERS = ["rf_types.hpp", "rf_math.hpp", "rf_internal.hpp"]
 ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")
 
@@ -29,10 +39,10 @@ FLAGS = [
 def build(verbose=False, jobs=4, defines=(), out=None, build_dir=None):
     bdir = build_dir or os.path.join(CSRC, "_build")
     os.makedirs(bdir, exist_ok=True)
-    objs = [os.path.join(bdir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    objs = [os.path.join(bdir, name + ".o") for name, _, _ in SOURCES]
     procs = []
-    for src, obj in zip(SOURCES, objs):
-        cmd = [HIPCC] + FLAGS + [f"-D{d}" for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
+    for (_, src, extra), obj in zip(SOURCES, objs):
+        cmd = [HIPCC] + FLAGS + list(extra) + [f"-D{d}" for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

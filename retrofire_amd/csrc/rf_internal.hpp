@@ -99,5 +99,8 @@ hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_gr
 // Same, with an event recorded after each stage (for per-kernel timing);
 // `ev` holds 8 events.
 hipError_t launch_pipeline_timed(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev);
+// The rasterizer's launch alone: rf_kernels.hip compiled with RF_RASTER_TU
+// (its own object, scheduled for instruction-level parallelism).
+hipError_t launch_raster(const BatchDesc& B, hipStream_t s, uint32_t raster_grid);
 
 }  // namespace rf

@@ -13,11 +13,19 @@
 //              per distinct tile the group touches.
 //   k_binsort  one workgroup per tile: sorts the tile's runs by group, which
 //              restores primitive submission order inside the tile.
-//   k_raster   one wave per tile (persistent): the tile's colour and depth
-//              live in LDS; each lane owns one tile row and walks that row's
-//              spans in submission order (tri_fill/scan raster.rs:185-302,
-//              ScanlineIter::next raster.rs:84-113, rasterize_fb
-//              target.rs:163-198), then the tile is written to HBM once.
+//   k_raster   one wave per tile (persistent): the tile's depth and colour
+//              sources live in LDS; a lane walks a triangle's edges, then
+//              lanes take its row spans in submission order and apply
+//              pixels under ordered (row, 8-column) cell claims (tri_fill/scan
+//              raster.rs:185-302, ScanlineIter::next raster.rs:84-113,
+//              rasterize_fb target.rs:163-198); the tile is written to HBM
+//              once.
+//
+// The file is compiled twice: as the pipeline (every kernel but k_raster, and
+// the launchers), and with RF_RASTER_TU defined as k_raster alone, under the
+// max-ilp scheduler. That scheduling shortens k_raster's dependent chains
+// (1.145 -> 1.131 ms per 64 crates1080 frames) and would lengthen k_setup's
+// (0.092 -> 0.096 ms), so it applies to k_raster only.
 //
 // Exactness: the serial recurrences of the reference (edge stepping per row,
 // varying stepping per pixel) are reproduced as serial sums; no closed forms.
@@ -142,6 +150,7 @@ __device__ __forceinline__ uint32_t find_draw_vert(const DrawDesc* d, uint32_t n
 // ---------------------------------------------------------------------------
 // k_vertex
 // ---------------------------------------------------------------------------
+#ifndef RF_RASTER_TU
 __global__ __launch_bounds__(256) void k_vertex(BatchDesc B) {
     const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= B.n_verts) return;
@@ -180,6 +189,7 @@ __global__ __launch_bounds__(256) void k_vertex(BatchDesc B) {
     s[4] = persp ? a1 / w : a1;
     s[5] = persp ? a2 / w : a2;
 }
+#endif  // RF_RASTER_TU
 
 // ---------------------------------------------------------------------------
 // k_setup helpers: clipping in registers
@@ -468,6 +478,7 @@ __device__ uint32_t edge_setup(const BatchDesc& B, const DrawDesc* D, const Fram
 // ---------------------------------------------------------------------------
 // k_setup
 // ---------------------------------------------------------------------------
+#ifndef RF_RASTER_TU
 __global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
     const uint32_t lane = lane_id();
     const uint32_t group = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
@@ -604,6 +615,7 @@ __global__ __launch_bounds__(256) void k_setup(BatchDesc B) {
     for_each_tile(has_rect, rect[0], rect[1], base, ntx,
                   [&](uint32_t tile, unsigned long long) { atomicAdd(&B.run_count[tile], 1u); });
 }
+#endif  // RF_RASTER_TU
 
 __device__ __forceinline__ uint32_t find_frame_tile(const FrameDesc* f, uint32_t n, uint32_t t) {
     uint32_t lo = 0, hi = n;
@@ -621,6 +633,7 @@ __device__ __forceinline__ uint32_t find_frame_tile(const FrameDesc* f, uint32_t
 // Two scans side by side: run counts (offsets of the tiles' run lists) and
 // "has runs" flags: B.tile_list lists the tiles with runs first, then the
 // others (which the rasterizer only clears).
+#ifndef RF_RASTER_TU
 __global__ __launch_bounds__(1024) void k_scan_local(BatchDesc B) {
     __shared__ uint32_t wave_tot[2][16];
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
@@ -707,6 +720,7 @@ __global__ __launch_bounds__(1024) void k_scan_add(BatchDesc B) {
     B.tile_list[B.run_count[i] != 0 ? busy_before : *B.busy_count + (i - busy_before)] = i;
     B.run_cursor[i] = 0;
 }
+#endif  // RF_RASTER_TU
 
 #ifndef RF_SORT_LDS
 #define RF_SORT_LDS 1024
@@ -746,6 +760,7 @@ __device__ __forceinline__ void wave_sort_runs(uint32_t n, uint32_t& group, unsi
 // itself, in registers). One wave per SORT_TPW tiles, whose long lists it
 // sorts one after another: bins of up to SORT_LDS runs in LDS, larger ones in
 // place in global memory.
+#ifndef RF_RASTER_TU
 __global__ __launch_bounds__(64) void k_binsort(BatchDesc B) {
     __shared__ unsigned long long key[SORT_LDS];
     __shared__ unsigned long long msk[SORT_LDS];
@@ -829,6 +844,7 @@ __global__ __launch_bounds__(64) void k_binsort(BatchDesc B) {
         }
     }
 }
+#endif  // RF_RASTER_TU
 
 // ---------------------------------------------------------------------------
 // k_raster
@@ -1030,6 +1046,7 @@ __device__ __forceinline__ void clear_empty_tiles(const BatchDesc& B, uint32_t w
 
 // Blocks past the groups' clear the tiles without runs: binning waits on
 // atomics and loads while the clears only store, so they share the GPU well.
+#ifndef RF_RASTER_TU
 __global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
     const uint32_t lane = lane_id();
     const uint32_t bin_blocks = (B.n_groups + 3) / 4;
@@ -1094,6 +1111,7 @@ __global__ __launch_bounds__(256) void k_bin(BatchDesc B) {
     }
     flush();
 }
+#endif  // RF_RASTER_TU
 // Per-triangle data of a batch slot (64 B).
 struct SlotP {
     float dv[2][4];        // d(z, a0, a1, a2)/dx of the upper and lower trapezoid
@@ -1213,6 +1231,7 @@ __device__ __forceinline__ bool depth_pass_u(int32_t dt, float nz, float cur) {
            ((dt == DT_GREATER) & (cur > nz));
 }
 
+#ifdef RF_RASTER_TU
 template <int TW>
 __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
@@ -1897,6 +1916,18 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     clear_empties();
 }
 
+hipError_t launch_raster(const BatchDesc& B, hipStream_t s, uint32_t raster_grid) {
+    if (B.tile_w == 16)
+        hipLaunchKernelGGL(k_raster<16>, dim3(raster_grid), dim3(64), 0, s, B);
+    else if (B.tile_w == 32)
+        hipLaunchKernelGGL(k_raster<32>, dim3(raster_grid), dim3(64), 0, s, B);
+    else
+        hipLaunchKernelGGL(k_raster<64>, dim3(raster_grid), dim3(64), 0, s, B);
+    return hipGetLastError();
+}
+#endif  // RF_RASTER_TU
+
+#ifndef RF_RASTER_TU
 // ---------------------------------------------------------------------------
 // Depth sort (render.rs:180-183, 209-219), before to_screen and face culling:
 //   k_fans        one thread per triangle: assembly and clipping as k_setup
@@ -2076,12 +2107,8 @@ static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raste
     mark(4);
     hipLaunchKernelGGL(k_binsort, dim3((B.n_tiles + SORT_TPW - 1) / SORT_TPW), dim3(WAVE), 0, s, B);
     mark(5);
-    if (B.tile_w == 16)
-        hipLaunchKernelGGL(k_raster<16>, dim3(raster_grid), dim3(64), 0, s, B);
-    else if (B.tile_w == 32)
-        hipLaunchKernelGGL(k_raster<32>, dim3(raster_grid), dim3(64), 0, s, B);
-    else
-        hipLaunchKernelGGL(k_raster<64>, dim3(raster_grid), dim3(64), 0, s, B);
+    chk(hipGetLastError());
+    chk(launch_raster(B, s, raster_grid));
     mark(6);
     chk(hipGetLastError());
     return e;
@@ -2109,5 +2136,6 @@ hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_gr
 hipError_t launch_pipeline_timed(const BatchDesc& B, hipStream_t s, uint32_t raster_grid, hipEvent_t* ev) {
     return launch_impl(B, s, raster_grid, ev);
 }
+#endif  // RF_RASTER_TU
 
 }  // namespace rf
