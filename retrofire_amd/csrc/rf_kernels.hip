@@ -1236,7 +1236,8 @@ template <int TW>
 __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
     constexpr int NPX = TW * LD;
-    constexpr int NCG = TW / CELL_W;
+    constexpr int CW = CELL_W < TW ? CELL_W : TW;  // ordering cell width in this tile
+    constexpr int NCG = TW / CW;
     __shared__ __align__(16) float s_dep[NPX];
     __shared__ uint32_t s_pay[NPX];                 // the pixel's colour source: payload
     __shared__ __align__(16) uint16_t s_tag[NPX];   //   and tag
@@ -1724,10 +1725,10 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
             // claim every cell of the remaining pixels
             int32_t cl0 = 0, cl1 = -1, cw1 = -1;
             if (have) {
-                cl0 = (xs + jc - (int32_t)x0) / CELL_W;
-                cl1 = (xs + j1 - 1 - (int32_t)x0) / CELL_W;
+                cl0 = (xs + jc - (int32_t)x0) / CW;
+                cl1 = (xs + j1 - 1 - (int32_t)x0) / CW;
                 const int32_t je = jc + CHUNK < j1 ? jc + CHUNK : j1;
-                cw1 = (xs + je - 1 - (int32_t)x0) / CELL_W;
+                cw1 = (xs + je - 1 - (int32_t)x0) / CW;
 #pragma unroll
                 for (int32_t c = 0; c < NCG; ++c)
                     if (c >= cl0 && c <= cl1) atomicMin(&s_cell[row * NCG + c], key);
