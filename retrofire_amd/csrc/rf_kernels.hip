@@ -1246,7 +1246,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     __shared__ EdgeRec s_rec[REC_CAP];
     __shared__ uint32_t s_fq[FQ_CAP];  // g << 3 | fan
     __shared__ int8_t s_own[REC_CAP];  // slot whose records start here, or -1
-    __shared__ uint32_t s_cell[TILE_H * NCG];
+    __shared__ __align__(16) uint32_t s_cell[TILE_H * NCG];
 
     const uint32_t lane = lane_id();
     for (uint32_t i = lane; i < (uint32_t)(TILE_H * NCG); i += WAVE) s_cell[i] = 0xFFFFFFFFu;
@@ -1724,6 +1724,8 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
             if (busy == 0ull) continue;
             // claim every cell of the remaining pixels
             int32_t cl0 = 0, cl1 = -1, cw1 = -1;
+            // a lane without a record reads and frees row 0's cells: harmless
+            const uint32_t crow = have ? row : 0u;
             if (have) {
                 cl0 = (xs + jc - (int32_t)x0) / CW;
                 cl1 = (xs + j1 - 1 - (int32_t)x0) / CW;
@@ -1733,10 +1735,14 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                 for (int32_t c = 0; c < NCG; ++c)
                     if (c >= cl0 && c <= cl1) atomicMin(&s_cell[row * NCG + c], key);
             }
+            // the row's cells in one (vector) read, after every lane's claims
+            uint32_t cells[NCG];
+#pragma unroll
+            for (int32_t c = 0; c < NCG; ++c) cells[c] = s_cell[crow * NCG + c];
             bool win = have;
 #pragma unroll
             for (int32_t c = 0; c < NCG; ++c)
-                if (c >= cl0 && c <= cw1) win = win && s_cell[row * NCG + c] == key;
+                if (c >= cl0 && c <= cw1) win = win && cells[c] == key;
             if (win) {
                 const int32_t rem = j1 - jc;
                 const uint32_t li = (uint32_t)(xs + jc - (int32_t)x0) * LD + row;
@@ -1840,10 +1846,11 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                 jc += CHUNK;
                 if (jc >= j1) have = false;
             }
-            // release the claims (every claimant resets what it touched)
+            // release the claims: every lane frees its whole row (one vector
+            // write). Every claimed cell lies in a claimant's row, so all
+            // cells are free again, as each round needs them at its start.
 #pragma unroll
-            for (int32_t c = 0; c < NCG; ++c)
-                if (c >= cl0 && c <= cl1) s_cell[row * NCG + c] = 0xFFFFFFFFu;
+            for (int32_t c = 0; c < NCG; ++c) s_cell[crow * NCG + c] = 0xFFFFFFFFu;
                 }
         __syncthreads();
 
