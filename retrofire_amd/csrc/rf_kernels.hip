@@ -1238,8 +1238,13 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     constexpr int NPX = TW * LD;
     constexpr int CW = CELL_W < TW ? CELL_W : TW;  // ordering cell width in this tile
     constexpr int NCG = TW / CW;
-    __shared__ __align__(16) float s_dep[NPX];
-    __shared__ uint32_t s_pay[NPX];                 // the pixel's colour source: payload
+    // depth, then the payloads, in one array: a chunk's depth reads run up to
+    // 7 columns past the last one, into the payloads (values never used), so
+    // that they are unconditional and pair up (ds_read2)
+    __shared__ __align__(16) float s_px[2 * NPX];
+    float* const s_dep = s_px;
+    uint32_t* const s_pay = reinterpret_cast<uint32_t*>(s_px + NPX);  // the pixel's colour source: payload
+    static_assert((CHUNK - 1) * LD + TILE_H <= NPX, "depth over-reads stay in the array");
     __shared__ __align__(16) uint16_t s_tag[NPX];   //   and tag
     static_assert((NPX * 4) % 16 == 0 && (NPX * 2) % 16 == 0, "16-byte LDS fills");
     __shared__ SlotP s_slot[WAVE];
@@ -1737,8 +1742,19 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
             }
             // the row's cells in one (vector) read, after every lane's claims
             uint32_t cells[NCG];
+            if constexpr (NCG % 4 == 0) {
 #pragma unroll
-            for (int32_t c = 0; c < NCG; ++c) cells[c] = s_cell[crow * NCG + c];
+                for (int32_t c = 0; c < NCG; c += 4) {
+                    const uint4 q = *reinterpret_cast<const uint4*>(&s_cell[crow * NCG + c]);
+                    cells[c] = q.x;
+                    cells[c + 1] = q.y;
+                    cells[c + 2] = q.z;
+                    cells[c + 3] = q.w;
+                }
+            } else {
+#pragma unroll
+                for (int32_t c = 0; c < NCG; ++c) cells[c] = s_cell[crow * NCG + c];
+            }
             bool win = have;
 #pragma unroll
             for (int32_t c = 0; c < NCG; ++c)
@@ -1762,7 +1778,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                 // the pixels of one chunk are distinct: read all depths first
                 if (has_depth) {
 #pragma unroll
-                    for (int p = 0; p < CHUNK; ++p) cur[p] = s_dep[p < rem ? li + p * LD : li];
+                    for (int p = 0; p < CHUNK; ++p) cur[p] = s_px[li + p * LD];  // p >= rem: unused
                 } else {
 #pragma unroll
                     for (int p = 0; p < CHUNK; ++p) cur[p] = 0.0f;
