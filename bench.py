@@ -69,15 +69,15 @@ def calib_copy(mib):
         raise RuntimeError(f"hipMemcpy failed ({rc})")
 
 
-def measured_traffic(tile_w):
+def measured_traffic(tile_w, chunk):
     """HBM bytes per k_raster launch from the committed counter summary, if it
-    was taken for this tile width (profiles/k_raster_traffic.json)."""
+    was taken for this tile width and span chunk (profiles/k_raster_traffic.json)."""
     try:
         with open(os.path.join(ROOT, "profiles", "k_raster_traffic.json")) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t.get("bytes_per_launch") if t.get("tile_w") == tile_w else None
+    return t.get("bytes_per_launch") if (t.get("tile_w"), t.get("chunk")) == (tile_w, chunk) else None
 
 
 def camera_frames(n, w, h, seed=12345):
@@ -160,6 +160,7 @@ def main():
         "config": {"workload": "crates1080: 1000 textured crates (24k verts, 12k tris, 256x256 RGBA8, "
                                "SamplerRepeatPot), 1920x1080 RGBA8 + f32 depth, clear per frame",
                    "frames_per_step": a.frames, "tile_w": batch.tile_w, "tile_h": int(_ffi.lib().rf_tile_height()),
+                   "chunk": batch.chunk,
                    "parallelism": "single GPU"},
         "mfrag_per_s": round(mfrag, 1),
         "frags_i_per_frame": frags_i // a.frames,
@@ -167,7 +168,7 @@ def main():
         "stage_ms": {n: round(x, 4) for n, x in zip(names, prof)},
         "roofline": {"bound": "hbm", "kernel": "k_raster", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(batch.tile_w),
+                     "traffic": measured_traffic(batch.tile_w, batch.chunk),
                      "alg_bytes_per_launch": alg_bytes, "launch_ms": round(raster_ms, 4)},
         "pipeline_alg_GBs": round(alg_frame_bytes * a.frames / (total_prof_ms * 1e-3) / 1e9, 1),
     }

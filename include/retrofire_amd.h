@@ -165,6 +165,11 @@ uint64_t rf_device_bytes(const rf_device* dev);
  * rf_render on this device. No counterpart in the reference (a tuning knob of
  * this implementation); results are identical for every width. */
 rf_status rf_device_set_tile_width(rf_device* dev, uint32_t tile_w);
+/* Rasterizer span chunk (6 or 8 pixels a lane applies per round; 0 = the
+ * default, 8) used by rf_render and by batches created with a forced tile
+ * width. No counterpart in the reference (a tuning knob of this
+ * implementation); results are identical for both. */
+rf_status rf_device_set_raster_chunk(rf_device* dev, uint32_t chunk);
 /* Rasterizer tile height (rows) this library was built with. No counterpart
  * in the reference (an implementation parameter, reported by benchmarks). */
 uint32_t rf_tile_height(void);
@@ -235,7 +240,9 @@ typedef struct rf_frame {
 
 typedef struct rf_batch rf_batch;
 /* Plans a batch: sizes scratch from one eager run, picks the tile width
- * (tile_w = 0: autotune; 16, 32 or 64 to force) and captures the launch sequence
+ * and span chunk (tile_w = 0: autotune over widths 16, 32, 64 and chunks 6,
+ * 8; 16, 32 or 64 to force the width, with the device's chunk, see
+ * rf_device_set_raster_chunk) and captures the launch sequence
  * into a HIP graph. The frames' descriptors are copied; the meshes, textures
  * and framebuffers must stay alive while the batch exists. */
 rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_frames,
@@ -246,6 +253,7 @@ rf_status rf_batch_run(rf_batch* b, int sync);
 rf_status rf_batch_wait(rf_batch* b, rf_stats* stats);
 rf_status rf_batch_free(rf_batch* b);
 uint32_t rf_batch_tile_width(const rf_batch* b);
+uint32_t rf_batch_raster_chunk(const rf_batch* b);
 /* Kernel-level timing of the last replay (HIP events on the batch stream):
  * total milliseconds and the rasterizer's share. */
 rf_status rf_batch_last_ms(const rf_batch* b, float* total_ms, float* raster_ms);

@@ -72,6 +72,7 @@ SIGNATURES = {
     "rf_status_str": (C.c_char_p, [_st]),
     "rf_device_bytes": (C.c_uint64, [_vp]),
     "rf_device_set_tile_width": (_st, [_vp, C.c_uint32]),
+    "rf_device_set_raster_chunk": (_st, [_vp, C.c_uint32]),
     "rf_tile_height": (C.c_uint32, []),
     "rf_render": (_st, [_vp, C.POINTER(RfDraw), C.POINTER(RfTarget), C.POINTER(RfCtx), C.POINTER(RfStats)]),
     "rf_mesh_create": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.c_uint64,
@@ -92,6 +93,7 @@ SIGNATURES = {
     "rf_batch_wait": (_st, [_vp, C.POINTER(RfStats)]),
     "rf_batch_free": (_st, [_vp]),
     "rf_batch_tile_width": (C.c_uint32, [_vp]),
+    "rf_batch_raster_chunk": (C.c_uint32, [_vp]),
     "rf_batch_last_ms": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "rf_batch_profile": (_st, [_vp, C.c_int, C.POINTER(C.c_float)]),
     "rf_serve_open": (_st, [_vp, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),

@@ -139,6 +139,11 @@ class Batch:
     def tile_w(self):
         return int(_ffi.lib().rf_batch_tile_width(self.h))
 
+    @property
+    def chunk(self):
+        """The rasterizer's span chunk (6 or 8) this batch runs with."""
+        return int(_ffi.lib().rf_batch_raster_chunk(self.h))
+
     def run(self, sync=True):
         check(_ffi.lib().rf_batch_run(self.h, int(sync)))
 

@@ -272,6 +272,12 @@ rf_status rf_device_set_tile_width(rf_device* dev, uint32_t tile_w) {
     return RF_OK;
 }
 
+rf_status rf_device_set_raster_chunk(rf_device* dev, uint32_t chunk) {
+    if (!dev || (chunk != 0 && chunk != 6 && chunk != 8)) return RF_E_ARG;
+    reinterpret_cast<Device*>(dev)->plain_chunk = chunk ? chunk : 8;
+    return RF_OK;
+}
+
 rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, const rf_ctx* ctx,
                     rf_stats* stats) {
     const double t0 = now_s();
@@ -372,6 +378,7 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     B.n_groups = (uint32_t)((draw->n_tris + WAVE_SIZE - 1) / WAVE_SIZE);
     B.n_tiles = tile_base;
     B.tile_w = tw;
+    B.chunk = d.plain_chunk;
     B.tile_h = TILE_ROWS;
     B.extra_cap = std::max<uint32_t>(d.plain_extra_cap, 1024u);
     B.runs_cap = std::max<uint32_t>(d.plain_runs_cap, 4 * B.n_groups + B.n_tiles + 1024);

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "../../include/retrofire_amd.h"
@@ -29,6 +30,7 @@ struct Batch {
     Scratch sc;
     BatchDesc B{};
     uint32_t tile_w = 64;
+    uint32_t chunk = 8;  // rasterizer span chunk (6 or 8)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -37,7 +39,7 @@ struct Batch {
     bool pending = false;
 };
 
-static rf_status build_descs(Batch& b, uint32_t tile_w) {
+static rf_status build_descs(Batch& b, uint32_t tile_w, uint32_t chunk) {
     Device& d = *b.dev;
     b.h_draws.clear();
     b.h_frames.clear();
@@ -119,10 +121,12 @@ static rf_status build_descs(Batch& b, uint32_t tile_w) {
     B.any_cull = any_cull;
     B.tile_w = tile_w;
     B.tile_h = TILE_ROWS;
+    B.chunk = chunk;
     B.extra_cap = std::max<uint32_t>(b.B.extra_cap, 1024u);
     B.runs_cap = std::max<uint32_t>(b.B.runs_cap, 4 * B.n_groups + B.n_tiles + 1024);
     b.B = B;
     b.tile_w = tile_w;
+    b.chunk = chunk;
     return RF_OK;
 }
 
@@ -224,12 +228,18 @@ rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_fra
         batch_release(b);
         return RF_E_DEVICE;
     }
-    // Tile width: forced, or the faster of the candidates on this batch.
-    std::vector<uint32_t> cands = tile_w ? std::vector<uint32_t>{tile_w} : std::vector<uint32_t>{16, 32, 64};
+    // Tile width and span chunk: the tile width forced (with the device's
+    // chunk), or the fastest (width, chunk) pair on this batch.
+    std::vector<std::pair<uint32_t, uint32_t>> cands;
+    if (tile_w)
+        cands.push_back({tile_w, d.plain_chunk});
+    else
+        for (uint32_t w : {16u, 32u, 64u})
+            for (uint32_t c : {6u, 8u}) cands.push_back({w, c});
     float best = 1e30f;
-    uint32_t best_w = cands[0];
-    for (uint32_t w : cands) {
-        if ((st = build_descs(*b, w)) || (st = size_batch(*b))) {
+    std::pair<uint32_t, uint32_t> best_wc = cands[0];
+    for (const auto& wc : cands) {
+        if ((st = build_descs(*b, wc.first, wc.second)) || (st = size_batch(*b))) {
             batch_release(b);
             return st;
         }
@@ -241,11 +251,12 @@ rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_fra
             }
             if (ms < best) {
                 best = ms;
-                best_w = w;
+                best_wc = wc;
             }
         }
     }
-    if (best_w != b->tile_w && ((st = build_descs(*b, best_w)) || (st = size_batch(*b)))) {
+    if ((best_wc.first != b->tile_w || best_wc.second != b->chunk) &&
+        ((st = build_descs(*b, best_wc.first, best_wc.second)) || (st = size_batch(*b)))) {
         batch_release(b);
         return st;
     }
@@ -311,6 +322,10 @@ rf_status rf_batch_free(rf_batch* bh) {
 
 uint32_t rf_batch_tile_width(const rf_batch* bh) {
     return bh ? reinterpret_cast<const Batch*>(bh)->tile_w : 0;
+}
+
+uint32_t rf_batch_raster_chunk(const rf_batch* bh) {
+    return bh ? reinterpret_cast<const Batch*>(bh)->chunk : 0;
 }
 
 rf_status rf_batch_last_ms(const rf_batch* bh, float* total_ms, float* raster_ms) {

@@ -47,6 +47,7 @@ struct Device {
     DevBuf ds_fans, ds_keys, ds_vals, ds_tmp, ds_idx;  // depth sort: clip fans, keys, order, sort scratch
     uint32_t ds_cap = 0;
     uint32_t plain_tile_w = 32;  // rasterizer tile width of rf_render
+    uint32_t plain_chunk = 8;    // rasterizer span chunk of rf_render and of batches with a forced tile width
     uint32_t plain_extra_cap = 0, plain_runs_cap = 0;
 };
 

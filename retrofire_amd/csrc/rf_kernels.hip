@@ -45,11 +45,7 @@ constexpr int BIN_MARGIN = 2;  // extra pixels around a triangle's bounding box
 #ifndef RF_FF_MIN
 #define RF_FF_MIN 12
 #endif
-#ifndef RF_CHUNK
-#define RF_CHUNK 8
-#endif
 constexpr int FF_MIN = RF_FF_MIN;  // step counts above this use the exact fast-forward
-constexpr int CHUNK = RF_CHUNK;    // pixels per lane processed together in a span
 
 // ---------------------------------------------------------------------------
 // wave helpers
@@ -1232,7 +1228,10 @@ __device__ __forceinline__ bool depth_pass_u(int32_t dt, float nz, float cur) {
 }
 
 #ifdef RF_RASTER_TU
-template <int TW>
+// TW: tile width (16, 32, 64); CH: pixels per lane applied together in a span
+// round (6 or 8: short spans waste fewer lanes' slots and registers at 6,
+// long spans take fewer rounds at 8).
+template <int TW, int CH>
 __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
     constexpr int NPX = TW * LD;
@@ -1244,7 +1243,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     __shared__ __align__(16) float s_px[2 * NPX];
     float* const s_dep = s_px;
     uint32_t* const s_pay = reinterpret_cast<uint32_t*>(s_px + NPX);  // the pixel's colour source: payload
-    static_assert((CHUNK - 1) * LD + TILE_H <= NPX, "depth over-reads stay in the array");
+    static_assert((CH - 1) * LD + TILE_H <= NPX, "depth over-reads stay in the array");
     __shared__ __align__(16) uint16_t s_tag[NPX];   //   and tag
     static_assert((NPX * 4) % 16 == 0 && (NPX * 2) % 16 == 0, "16-byte LDS fills");
     __shared__ SlotP s_slot[WAVE];
@@ -1734,7 +1733,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
             if (have) {
                 cl0 = (xs + jc - (int32_t)x0) / CW;
                 cl1 = (xs + j1 - 1 - (int32_t)x0) / CW;
-                const int32_t je = jc + CHUNK < j1 ? jc + CHUNK : j1;
+                const int32_t je = jc + CH < j1 ? jc + CH : j1;
                 cw1 = (xs + je - 1 - (int32_t)x0) / CW;
 #pragma unroll
                 for (int32_t c = 0; c < NCG; ++c)
@@ -1762,10 +1761,10 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
             if (win) {
                 const int32_t rem = j1 - jc;
                 const uint32_t li = (uint32_t)(xs + jc - (int32_t)x0) * LD + row;
-                float zz[CHUNK], v0[CHUNK], v1[CHUNK], v2[CHUNK], cur[CHUNK];
-                bool ps[CHUNK];
+                float zz[CH], v0[CH], v1[CH], v2[CH], cur[CH];
+                bool ps[CH];
 #pragma unroll
-                for (int p = 0; p < CHUNK; ++p) {
+                for (int p = 0; p < CH; ++p) {
                     zz[p] = z;
                     v0[p] = c0;
                     v1[p] = c1;
@@ -1778,14 +1777,14 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                 // the pixels of one chunk are distinct: read all depths first
                 if (has_depth) {
 #pragma unroll
-                    for (int p = 0; p < CHUNK; ++p) cur[p] = s_px[li + p * LD];  // p >= rem: unused
+                    for (int p = 0; p < CH; ++p) cur[p] = s_px[li + p * LD];  // p >= rem: unused
                 } else {
 #pragma unroll
-                    for (int p = 0; p < CHUNK; ++p) cur[p] = 0.0f;
+                    for (int p = 0; p < CH; ++p) cur[p] = 0.0f;
                 }
                 uint32_t npass = 0;
 #pragma unroll
-                for (int p = 0; p < CHUNK; ++p) {
+                for (int p = 0; p < CH; ++p) {
                     ps[p] = (p < rem) & depth_pass_u(dtest, zz[p], cur[p]);
                     npass += ps[p] ? 1u : 0u;
                 }
@@ -1793,7 +1792,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                     // the shader runs for every passing fragment, colour write or not
                     const float tw = (float)ip0, th = (float)ip1;
 #pragma unroll
-                    for (int p = 0; p < CHUNK; ++p)
+                    for (int p = 0; p < CH; ++p)
                         if (ps[p] && (cvt_u32(tw * (v0[p] / zz[p])) >= ip0 || cvt_u32(th * (v1[p] / zz[p])) >= ip1))
                             errs |= ERRB_TEXEL;
                 }
@@ -1801,16 +1800,16 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                     fo_acc += npass;
                     // colour sources of the chunk's pixels (the shader kind is
                     // decided once per chunk)
-                    uint32_t pay[CHUNK];
+                    uint32_t pay[CH];
                     uint32_t tg;
                     if (ikind == KIND_FLAT) {
                         tg = TAG_COLOR;
 #pragma unroll
-                        for (int p = 0; p < CHUNK; ++p) pay[p] = ip0;
+                        for (int p = 0; p < CH; ++p) pay[p] = ip0;
                     } else if (ikind == KIND_GOURAUD) {  // Color3f::to_color4 (color.rs:253-263)
                         tg = TAG_COLOR;
 #pragma unroll
-                        for (int p = 0; p < CHUNK; ++p)
+                        for (int p = 0; p < CH; ++p)
                             pay[p] = pack_pixel(F.fmt, cvt_u8(256.0f * v0[p]), cvt_u8(256.0f * v1[p]),
                                                 cvt_u8(256.0f * v2[p]), 0xFF);
                     } else {
@@ -1819,7 +1818,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                         if (ikind == KIND_TEX_CLAMP) {  // SamplerClamp (tex.rs:280-303)
                             const float mu = tw - 1.0f, mv = th - 1.0f;
 #pragma unroll
-                            for (int p = 0; p < CHUNK; ++p) {
+                            for (int p = 0; p < CH; ++p) {
                                 // Scanline::fragments z_div (raster.rs:61-68, vec.rs:666-676)
                                 float su = (v0[p] / zz[p]) * tw, sv = (v1[p] / zz[p]) * th;
                                 su = su < 0.0f ? 0.0f : su;
@@ -1830,14 +1829,14 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                             }
                         } else if (ikind == KIND_TEX_REPEAT_POT) {  // SamplerRepeatPot (tex.rs:240-266)
 #pragma unroll
-                            for (int p = 0; p < CHUNK; ++p) {
+                            for (int p = 0; p < CH; ++p) {
                                 const uint32_t ui = (uint32_t)cvt_i32(floorf(tw * (v0[p] / zz[p]))) & (ip0 - 1);
                                 const uint32_t vi = (uint32_t)cvt_i32(floorf(th * (v1[p] / zz[p]))) & (ip1 - 1);
                                 pay[p] = vi * ip2 + ui;
                             }
                         } else {  // SamplerOnce (tex.rs:343-357): `as u32`, no wrap
 #pragma unroll
-                            for (int p = 0; p < CHUNK; ++p) {
+                            for (int p = 0; p < CH; ++p) {
                                 const uint32_t ui = cvt_u32(tw * (v0[p] / zz[p]));
                                 const uint32_t vi = cvt_u32(th * (v1[p] / zz[p]));
                                 // outside the texture the reference panics (checked
@@ -1847,7 +1846,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                         }
                     }
 #pragma unroll
-                    for (int p = 0; p < CHUNK; ++p) {
+                    for (int p = 0; p < CH; ++p) {
                         if (ps[p]) {
                             s_tag[li + p * LD] = (uint16_t)tg;
                             s_pay[li + p * LD] = pay[p];
@@ -1856,10 +1855,10 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                 }
                 if (dwrite) {
 #pragma unroll
-                    for (int p = 0; p < CHUNK; ++p)
+                    for (int p = 0; p < CH; ++p)
                         if (ps[p]) s_dep[li + p * LD] = zz[p];
                 }
-                jc += CHUNK;
+                jc += CH;
                 if (jc >= j1) have = false;
             }
             // release the claims: every lane frees its whole row (one vector
@@ -1941,12 +1940,23 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
 }
 
 hipError_t launch_raster(const BatchDesc& B, hipStream_t s, uint32_t raster_grid) {
-    if (B.tile_w == 16)
-        hipLaunchKernelGGL(k_raster<16>, dim3(raster_grid), dim3(64), 0, s, B);
-    else if (B.tile_w == 32)
-        hipLaunchKernelGGL(k_raster<32>, dim3(raster_grid), dim3(64), 0, s, B);
-    else
-        hipLaunchKernelGGL(k_raster<64>, dim3(raster_grid), dim3(64), 0, s, B);
+    const bool c6 = B.chunk == 6;
+    if (B.tile_w == 16) {
+        if (c6)
+            hipLaunchKernelGGL((k_raster<16, 6>), dim3(raster_grid), dim3(64), 0, s, B);
+        else
+            hipLaunchKernelGGL((k_raster<16, 8>), dim3(raster_grid), dim3(64), 0, s, B);
+    } else if (B.tile_w == 32) {
+        if (c6)
+            hipLaunchKernelGGL((k_raster<32, 6>), dim3(raster_grid), dim3(64), 0, s, B);
+        else
+            hipLaunchKernelGGL((k_raster<32, 8>), dim3(raster_grid), dim3(64), 0, s, B);
+    } else {
+        if (c6)
+            hipLaunchKernelGGL((k_raster<64, 6>), dim3(raster_grid), dim3(64), 0, s, B);
+        else
+            hipLaunchKernelGGL((k_raster<64, 8>), dim3(raster_grid), dim3(64), 0, s, B);
+    }
     return hipGetLastError();
 }
 #endif  // RF_RASTER_TU

@@ -318,6 +318,7 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     B.n_tiles = tile_base;
     B.tile_w = tile_w;
     B.tile_h = TILE_ROWS;
+    B.chunk = 8;  // span chunk of served launches
     B.extra_cap = l.extra_cap;
     B.runs_cap = std::max<uint32_t>(l.runs_cap, 8 * B.n_groups + B.n_tiles + 4096);
     if (l.sc.prepare(B, &s.bytes)) return RF_E_DEVICE;

@@ -154,6 +154,7 @@ struct BatchDesc {
     uint32_t n_groups;  // ceil(n_tris / 64)
     uint32_t n_tiles;   // total tiles over all frames
     uint32_t tile_w, tile_h;
+    uint32_t chunk;     // rasterizer span chunk: 6 or 8 pixels (anything else: 8)
     // scratch
     float4* cv;          // clip-space position per vertex slot
     float4* ca;          // clip attributes (a0, a1, a2, outcode bits)

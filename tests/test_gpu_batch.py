@@ -74,20 +74,25 @@ def test_batched_uneven_frames_match_oracle(world):
                               texture=tex),
                  RB.mesh_draw(icos, rf.shader.Gouraud(), f["mvp_i"], f["vp"], n_tris=f["ni"])]
         spec.append((fb, rf.Context(clear=True), draws))
-    for tw in (0, 16, 32, 64):
+    dev = rf.default_device()
+    # autotuned, then every forced tile width with each span chunk
+    for tw, ch in ((0, 0), (16, 8), (32, 8), (64, 8), (16, 6), (32, 6), (64, 6)):
+        dev.set_raster_chunk(ch)
         b = RB.Batch(spec, tile_w=tw)
+        assert b.chunk in (6, 8) and (tw == 0 or b.chunk == ch)
         for rep in range(3):  # graph replays give the same result every time
             b.run(sync=True)
             stats = b.wait()
             for k, f in enumerate(frames):
                 col, dep = fbs[k].read()
                 ocol, odep, ost = oracle_frame(world, f)
-                assert np.array_equal(col, ocol), f"frame {k} colour (tile_w {b.tile_w}, replay {rep})"
+                assert np.array_equal(col, ocol), f"frame {k} colour (tile_w {b.tile_w}, chunk {b.chunk}, replay {rep})"
                 assert np.array_equal(dep.view(np.uint32), odep.view(np.uint32)), f"frame {k} depth"
                 s = stats[k]
                 assert [s.prims_o, s.verts_o, s.frags_i, s.frags_o] == [int(x) for x in ost[[2, 4, 5, 6]]]
                 assert s.calls == 2
         b.free()
+    dev.set_raster_chunk(0)
     for fb in fbs:
         fb.free()
     crates.free()
@@ -235,6 +240,7 @@ def test_argument_checks():
     import ctypes as C
     dev = rf.default_device()
     assert _ffi.lib().rf_device_set_tile_width(dev.h, 48) == -3
+    assert _ffi.lib().rf_device_set_raster_chunk(dev.h, 7) == -3
     m = RB.Mesh(np.zeros((3, 3), np.float32), np.array([[0, 1, 2]], np.uint32))
     fb = RB.DeviceFramebuf(16, 16)
     d = RB.mesh_draw(m, rf.shader.Flat(), M.identity(), M.viewport(0, 16, 16, 0))

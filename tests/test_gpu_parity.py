@@ -17,13 +17,16 @@ from test_oracle_kats import GOLDEN, hello_tri_inputs, read_ppm, textured_quad_i
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=[16, 32, 64], ids=["tw16", "tw32", "tw64"])
+@pytest.fixture(autouse=True, params=[(16, 8), (32, 8), (64, 8), (16, 6), (32, 6), (64, 6)],
+                ids=["tw16", "tw32", "tw64", "tw16c6", "tw32c6", "tw64c6"])
 def tile_width(request):
-    """Every parity case runs with both rasterizer tile widths."""
+    """Every case runs with each rasterizer tile width and span chunk."""
     dev = rf.default_device()
-    dev.set_tile_width(request.param)
-    yield request.param
+    dev.set_tile_width(request.param[0])
+    dev.set_raster_chunk(request.param[1])
+    yield request.param[0]
     dev.set_tile_width(0)
+    dev.set_raster_chunk(0)
 
 
 def test_textured_quad_golden_gpu():  # core/tests/rendering.rs:18-60
