@@ -1238,8 +1238,8 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     constexpr int CW = CELL_W < TW ? CELL_W : TW;  // ordering cell width in this tile
     constexpr int NCG = TW / CW;
     // depth, then the payloads, in one array: a chunk's depth reads run up to
-    // 7 columns past the last one, into the payloads (values never used), so
-    // that they are unconditional and pair up (ds_read2)
+    // CH - 1 columns past the last one, into the payloads (values never
+    // used), so that they are unconditional and pair up (ds_read2)
     __shared__ __align__(16) float s_px[2 * NPX];
     float* const s_dep = s_px;
     uint32_t* const s_pay = reinterpret_cast<uint32_t*>(s_px + NPX);  // the pixel's colour source: payload
