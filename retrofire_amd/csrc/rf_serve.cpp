@@ -230,6 +230,12 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     std::vector<FrameDesc> fds;
     uint64_t vbase = 0, tbase = 0;
     uint32_t tile_base = 0;
+#ifndef RF_SERVE_CHUNK
+// 6-pixel span chunks: served launches mix meshes of small triangles, where
+// they shorten the latency, with larger ones, where they cost nothing
+// measurable
+#define RF_SERVE_CHUNK 6
+#endif
 #ifndef RF_SERVE_TILE_W
 #define RF_SERVE_TILE_W 32
 #endif
@@ -318,7 +324,7 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     B.n_tiles = tile_base;
     B.tile_w = tile_w;
     B.tile_h = TILE_ROWS;
-    B.chunk = 8;  // span chunk of served launches
+    B.chunk = RF_SERVE_CHUNK;  // span chunk of served launches
     B.extra_cap = l.extra_cap;
     B.runs_cap = std::max<uint32_t>(l.runs_cap, 8 * B.n_groups + B.n_tiles + 4096);
     if (l.sc.prepare(B, &s.bytes)) return RF_E_DEVICE;
