@@ -379,6 +379,7 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     B.n_tiles = tile_base;
     B.tile_w = tw;
     B.chunk = d.plain_chunk;
+    B.sort_tpw = 1;  // one frame: one tile per binsort wave costs little and spreads dense tiles
     B.tile_h = TILE_ROWS;
     B.extra_cap = std::max<uint32_t>(d.plain_extra_cap, 1024u);
     B.runs_cap = std::max<uint32_t>(d.plain_runs_cap, 4 * B.n_groups + B.n_tiles + 1024);

@@ -260,6 +260,19 @@ rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_fra
         batch_release(b);
         return st;
     }
+    // Then the tiles per binsort wave: 8 (the default) or 1 (dense scenes'
+    // long run lists spread over more waves).
+    if (cands.size() > 1) {
+        b->B.sort_tpw = 1;
+        const float ms1 = time_eager(*b, 3);
+        b->B.sort_tpw = 0;
+        const float ms8 = time_eager(*b, 3);
+        if (ms1 < 0 || ms8 < 0) {
+            batch_release(b);
+            return RF_E_DEVICE;
+        }
+        if (ms1 < ms8) b->B.sort_tpw = 1;
+    }
     // headroom, then the final scratch and the graph
     b->B.extra_cap += b->B.extra_cap / 8 + 64;
     b->B.runs_cap += b->B.runs_cap / 8 + 64;

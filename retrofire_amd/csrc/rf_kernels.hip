@@ -726,8 +726,14 @@ __global__ __launch_bounds__(1024) void k_scan_add(BatchDesc B) {
 #define RF_SORT_TPW 8
 #endif
 constexpr int SORT_LDS = RF_SORT_LDS;  // bins up to this many runs sort in LDS
-constexpr int SORT_TPW = RF_SORT_TPW;  // tiles per k_binsort wave (their sorts run one after another)
+constexpr int SORT_TPW = RF_SORT_TPW;  // default tiles per k_binsort wave (their sorts run one after another)
 static_assert(SORT_TPW >= 1 && SORT_TPW <= 64, "one lane per tile");
+// Tiles per k_binsort wave of this batch (1..64; 0: the default). One tile a
+// wave spreads dense scenes' long run lists over many waves; 8 a wave costs
+// fewer waves where lists are short.
+__host__ __device__ __forceinline__ uint32_t sort_tpw(const BatchDesc& B) {
+    return B.sort_tpw >= 1 && B.sort_tpw <= 64 ? B.sort_tpw : (uint32_t)SORT_TPW;
+}
 
 // Sorts the runs held one per lane (lanes >= n hold none) by group: a bitonic
 // network over the 64 lanes; lane i ends with the i-th run. Groups are
@@ -754,7 +760,7 @@ __device__ __forceinline__ void wave_sort_runs(uint32_t n, uint32_t& group, unsi
 
 // Orders the runs of each tile with more than 64 of them by group (k_bin
 // appends them in arrival order; the rasterizer sorts lists of up to 64 runs
-// itself, in registers). One wave per SORT_TPW tiles, whose long lists it
+// itself, in registers). One wave per sort_tpw(B) tiles, whose long lists it
 // sorts one after another: bins of up to SORT_LDS runs in LDS, larger ones in
 // place in global memory.
 #ifndef RF_RASTER_TU
@@ -762,10 +768,11 @@ __global__ __launch_bounds__(64) void k_binsort(BatchDesc B) {
     __shared__ unsigned long long key[SORT_LDS];
     __shared__ unsigned long long msk[SORT_LDS];
     const uint32_t lane = lane_id();
-    const uint32_t t0 = blockIdx.x * SORT_TPW;
+    const uint32_t tpw = sort_tpw(B);
+    const uint32_t t0 = blockIdx.x * tpw;
     const uint32_t t = t0 + lane;
     uint32_t off = 0, n = 0;
-    if (lane < (uint32_t)SORT_TPW && t < B.n_tiles) {
+    if (lane < tpw && t < B.n_tiles) {
         off = B.run_off[t];
         n = B.run_off[t + 1] - off;
         if (off >= B.runs_cap) n = 0;
@@ -2140,7 +2147,7 @@ static hipError_t launch_impl(const BatchDesc& B0, hipStream_t s, uint32_t raste
             hipLaunchKernelGGL(k_bin, dim3(bin_blocks + clear_blocks), dim3(256), 0, s, B);
     }
     mark(4);
-    hipLaunchKernelGGL(k_binsort, dim3((B.n_tiles + SORT_TPW - 1) / SORT_TPW), dim3(WAVE), 0, s, B);
+    hipLaunchKernelGGL(k_binsort, dim3((B.n_tiles + sort_tpw(B) - 1) / sort_tpw(B)), dim3(WAVE), 0, s, B);
     mark(5);
     chk(hipGetLastError());
     chk(launch_raster(B, s, raster_grid));

@@ -325,6 +325,7 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     B.tile_w = tile_w;
     B.tile_h = TILE_ROWS;
     B.chunk = RF_SERVE_CHUNK;  // span chunk of served launches
+    B.sort_tpw = 1;            // few frames a launch: one tile per binsort wave
     B.extra_cap = l.extra_cap;
     B.runs_cap = std::max<uint32_t>(l.runs_cap, 8 * B.n_groups + B.n_tiles + 4096);
     if (l.sc.prepare(B, &s.bytes)) return RF_E_DEVICE;

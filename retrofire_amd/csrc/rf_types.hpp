@@ -155,6 +155,7 @@ struct BatchDesc {
     uint32_t n_tiles;   // total tiles over all frames
     uint32_t tile_w, tile_h;
     uint32_t chunk;     // rasterizer span chunk: 6 or 8 pixels (anything else: 8)
+    uint32_t sort_tpw;  // tiles per k_binsort wave (1..64; 0: the default, 8)
     // scratch
     float4* cv;          // clip-space position per vertex slot
     float4* ca;          // clip attributes (a0, a1, a2, outcode bits)
