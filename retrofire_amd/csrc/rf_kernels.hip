@@ -538,15 +538,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                 if (n < 0) atomicOr(B.err, ERRB_CLIP);
                 if (n >= 3) {
                     const bool persp = kind_persp(D->kind);
-                    float sc[CLIP_MAX][6];
-                    for (int k = 0; k < n; ++k) to_screen(poly[k], D->vp, persp, sc[k]);
+                    // screen vertices in registers, two passes over the fan
+                    // (recomputed, same bits), rather than a per-lane array
+                    // in scratch memory
+                    float s0v[6], sa[6], sb[6];
+                    to_screen(poly[0], D->vp, persp, s0v);
                     // fan [p, e0, e1] over rest.windows(2) (clip.rs:373-396)
                     uint32_t keep = 0, cnt = 0;
-                    for (int k = 1; k + 1 < n; ++k)
-                        if (!culled(F.face_cull, backface(sc[0], sc[k], sc[k + 1]))) {
+                    to_screen(poly[1], D->vp, persp, sa);
+                    for (int k = 1; k + 1 < n; ++k) {
+                        to_screen(poly[k + 1], D->vp, persp, sb);
+                        if (!culled(F.face_cull, backface(s0v, sa, sb))) {
                             keep |= 1u << k;
                             ++cnt;
                         }
+                        for (int c = 0; c < 6; ++c) sa[c] = sb[c];
+                    }
                     if (cnt > 1) {
                         extra_base = atomicAdd(B.extra_count, cnt - 1);
                         if (extra_base + cnt - 1 > B.extra_cap) {
@@ -555,14 +562,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                         }
                     }
                     uint32_t j = 0;
+                    to_screen(poly[1], D->vp, persp, sa);
                     for (int k = 1; k + 1 < n && j < cnt; ++k) {
-                        if (!(keep & (1u << k))) continue;
-                        ScrTri* dst = (j == 0) ? &B.scr[g] : &B.extra[extra_base + j - 1];
-                        write_scr(dst, sc[0], sc[k], sc[k + 1], di, D->kind);
-                        ra.add(sc[0]);
-                        ra.add(sc[k]);
-                        ra.add(sc[k + 1]);
-                        ++j;
+                        to_screen(poly[k + 1], D->vp, persp, sb);
+                        if (keep & (1u << k)) {
+                            ScrTri* dst = (j == 0) ? &B.scr[g] : &B.extra[extra_base + j - 1];
+                            write_scr(dst, s0v, sa, sb, di, D->kind);
+                            ra.add(s0v);
+                            ra.add(sa);
+                            ra.add(sb);
+                            ++j;
+                        }
+                        for (int c = 0; c < 6; ++c) sa[c] = sb[c];
                     }
                     nf = cnt;
                 }
