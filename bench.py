@@ -38,9 +38,11 @@ def parse():
     p.add_argument("--tile-w", type=int, default=0, help="0 = autotune")
     p.add_argument("--serve-seconds", type=float, default=2.0)
     p.add_argument("--serve-rate", type=float, default=1000.0)
-    p.add_argument("--serve-sweep-seconds", type=float, default=1.0,
+    p.add_argument("--serve-sweep-seconds", type=float, default=5.0,
                    help="seconds per offered rate of the 16-mesh serving sweep (0 = no sweep)")
-    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--cpu-seconds", type=float, default=20.0)
+    p.add_argument("--c3-steps", type=int, default=10, help="timed launches of the 64-frame mixed batch")
+    p.add_argument("--no-c3", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--no-plain", action="store_true")
@@ -169,6 +171,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_raster", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(batch.tile_w, batch.chunk),
+                     "traffic_source": "committed rocprofv3 FETCH_SIZE + WRITE_SIZE measurement of this kernel "
+                                       "and workload (profiles/k_raster_traffic.json), not measured in this run",
                      "alg_bytes_per_launch": alg_bytes, "launch_ms": round(raster_ms, 4)},
         "pipeline_alg_GBs": round(alg_frame_bytes * a.frames / (total_prof_ms * 1e-3) / 1e9, 1),
     }
@@ -192,9 +196,14 @@ def main():
     if not a.no_serve and a.serve_seconds > 0:
         out["serving"] = serve_run(rf, RB, mesh, texr, shd, a)
 
-    # CPU baseline: the oracle on the same workload, one thread, bounded sample
+    # the batched workload at its stated size (64 uneven frames of the full
+    # crates and icosphere scenes)
+    if not a.no_c3:
+        out["batched_c3"] = c3_run(rf, RB, a, a.c3_steps)
+
+    # CPU baseline: the oracle on the same workloads, one pinned core, bounded sample
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pos, uv, tris, tex, mvps, vp, W, H, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         out["gpu_over_cpu"] = round(fps / out["cpu_baseline"]["value"], 1)
 
     batch.free()
@@ -205,27 +214,18 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-RES_LIST = [(256, 256), (512, 512), (1280, 720), (1920, 1080), (2048, 2048)]
+def crates_request(RB, mesh, shd, texr, w, h, yaw):
+    from retrofire_amd import scenes as S
+    from retrofire_amd import mat as M
+    return [RB.mesh_draw(mesh, shd, S.crates_mvp(w, h, yaw), M.viewport(0, h, w, 0), texture=texr)]
 
 
-def crates_request(S, M, RB, mesh, shd, texr, w, h, yaw):
-    aspect = np.float32(w) / np.float32(h)
-    proj = M.perspective(M.fov_diagonal_focal_ratio(90.0, aspect), aspect, 0.1, 1000.0)
-    mvp = M.then(S.first_person(0.0, 1.7, 0.0, yaw), proj)
-    return [RB.mesh_draw(mesh, shd, mvp, M.viewport(0, h, w, 0), texture=texr)]
-
-
-def spheres_request(S, M, RB, mesh, shd, w, h):
-    mvp = M.then(M.translate3(0.0, 0.0, 2.5), M.perspective(np.float32(1.7320508), np.float32(w) / np.float32(h),
-                                                            0.1, 100.0))
-    return [RB.mesh_draw(mesh, shd, mvp, M.viewport(0, h, w, 0))]
-
-
-def serve_stream(RB, reqs, seconds, warm):
+def serve_stream(RB, reqs, seconds, warm, budget=4 << 30):
     """Open-loop replay of (time, w, h, draws) requests into a fresh server;
-    latency = completion - submission. `warm` (w, h, draws) requests run
-    first, untimed, so buffer pools and scratch reach their steady size."""
-    srv = RB.Server(budget_bytes=4 << 30, max_batch=32, n_streams=3)
+    latency = completion - submission, wait = admission - submission. `warm`
+    (w, h, draws) requests run first, untimed, so the pinned readback pool
+    and the learned scratch ratios reach their steady state."""
+    srv = RB.Server(budget_bytes=budget, max_batch=32, n_streams=3)
     n = len(reqs)
     for k, (w, h, draws) in enumerate(warm):
         srv.submit(10 ** 9 + k, draws, w, h)
@@ -253,16 +253,21 @@ def serve_stream(RB, reqs, seconds, warm):
     if len(done) < n:
         return {"requests": n, "completed": len(done), "error": "serving run did not finish"}
     lat = np.array([(r.done_s - r.submit_s) for r in done]) * 1e3
+    wait = np.array([(r.start_s - r.submit_s) for r in done]) * 1e3
     ok = sum(1 for r in done if r.status == 0)
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3)  # noqa: E731
     return {"requests": n, "ok": ok, "frames_per_s": round(n / (t1 - t0), 1),
-            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+            "p50_ms": pct(lat, 50), "p90_ms": pct(lat, 90), "p99_ms": pct(lat, 99), "max_ms": round(float(lat.max()), 3),
+            "wait_p50_ms": pct(wait, 50), "wait_p99_ms": pct(wait, 99)}
 
 
 def serve_run(rf, RB, mesh, texr, shd, a):
     """Serving: open-loop Poisson arrivals. Headline at --serve-rate over the
     crates1080 mesh at the five resolutions; then a sweep of offered rates
-    over 16 resident meshes (8 crate fields of 125..1000 crates, 8 icosphere
-    clouds of 250..2000 spheres), each request one mesh at one resolution."""
+    over the 16 resident meshes of the served workload (8 crate fields of
+    125..1000 crates, 8 icosphere clouds of 250..2000 spheres), each request
+    one mesh at one resolution from a random pose, beside the plain
+    sequential rate of the same requests (one synchronous rf_render each)."""
     from retrofire_amd import mat as M
     from retrofire_amd import scenes as S
     rng = S.Xorshift64(S.DEFAULT_SEED + 4)
@@ -270,80 +275,172 @@ def serve_run(rf, RB, mesh, texr, shd, a):
     reqs, t = [], 0.0
     for i in range(n):
         t += -math.log(max(1e-7, 1.0 - float(rng.uniform_f32(0.0, 1.0)))) / a.serve_rate
-        w, h = RES_LIST[rng.uniform_usize(0, 5)]
-        reqs.append((t, w, h, crates_request(S, M, RB, mesh, shd, texr, w, h, float(rng.uniform_f32(0.0, 1.0)))))
-    out = serve_stream(RB, reqs, a.serve_seconds, [(w, h, reqs[0][3]) for (w, h) in RES_LIST])
+        w, h = S.RESOLUTIONS[rng.uniform_usize(0, 5)]
+        reqs.append((t, w, h, crates_request(RB, mesh, shd, texr, w, h, float(rng.uniform_f32(0.0, 1.0)))))
+    out = serve_stream(RB, reqs, a.serve_seconds, [(w, h, reqs[0][3]) for (w, h) in S.RESOLUTIONS])
     out["offered_rate"] = a.serve_rate
     out["resolutions"] = "uniform over 256^2, 512^2, 1280x720, 1920x1080, 2048^2"
     if a.serve_sweep_seconds <= 0:
         return out
-    # 16 resident meshes
-    meshes = []
-    for k in range(8):
-        pos, uv, tris, _ = S.crates_world(125 * (k + 1), seed=S.DEFAULT_SEED + 100 + k)
-        meshes.append(("crates", RB.Mesh(pos, tris, uv)))
+    meshes, tex_px = S.serve_meshes()
+    dmeshes = [RB.Mesh(p, t_, at) for (_, p, at, t_) in meshes]
     gour = rf.shader.Gouraud()
-    for k in range(8):
-        pos, attr, tris, _, _ = S.icospheres_world(250 * (k + 1), seed=S.DEFAULT_SEED + 200 + k)
-        meshes.append(("spheres", RB.Mesh(pos, tris, attr)))
+
+    def draws_of(w, h, mi, pose):
+        if meshes[mi][0] == "crates":
+            return crates_request(RB, dmeshes[mi], shd, texr, w, h, pose)
+        return [RB.mesh_draw(dmeshes[mi], gour, S.spheres_mvp(w, h, pose), M.viewport(0, h, w, 0))]
+
+    warm = [(w, h, draws_of(w, h, mi, 0.0)) for (w, h) in S.RESOLUTIONS for mi in (7, 15)]
     sweep = []
     for rate in (200.0, 500.0, 1000.0, 2000.0):
-        rng = S.Xorshift64(S.DEFAULT_SEED + 5)
-        m = max(1, int(rate * a.serve_sweep_seconds))
-        reqs, t = [], 0.0
-        for i in range(m):
-            t += -math.log(max(1e-7, 1.0 - float(rng.uniform_f32(0.0, 1.0)))) / rate
-            w, h = RES_LIST[rng.uniform_usize(0, 5)]
-            kind, me = meshes[rng.uniform_usize(0, 16)]
-            yaw = float(rng.uniform_f32(0.0, 1.0))
-            draws = (crates_request(S, M, RB, me, shd, texr, w, h, yaw) if kind == "crates"
-                     else spheres_request(S, M, RB, me, gour, w, h))
-            reqs.append((t, w, h, draws))
-        warm = [(w, h, crates_request(S, M, RB, meshes[7][1], shd, texr, w, h, 0.0)) for (w, h) in RES_LIST]
-        warm += [(w, h, spheres_request(S, M, RB, meshes[15][1], gour, w, h)) for (w, h) in RES_LIST]
+        spec = S.serve_stream(rate, max(1, int(rate * a.serve_sweep_seconds)), seed=5)
+        reqs = [(t_, w, h, draws_of(w, h, mi, pose)) for (t_, w, h, mi, pose) in spec]
         r = serve_stream(RB, reqs, a.serve_sweep_seconds, warm)
         r["offered_rate"] = rate
         sweep.append(r)
-    for _, me in meshes:
+    # the plain sequential GPU call on the same kind of requests
+    spec = S.serve_stream(1000.0, 200, seed=5)
+    col = np.zeros((2048, 2048, 4), np.uint8)
+    dep = np.zeros((2048, 2048), np.float32)
+    crates_shd = rf.shader.TexRepeatPot(tex_px)
+    t0 = time.perf_counter()
+    for (_, w, h, mi, pose) in spec:
+        kind, p, at, tr = meshes[mi]
+        c, d = col[:h, :w], dep[:h, :w]
+        if kind == "crates":
+            rf.render(tr, (p, at), crates_shd, S.crates_mvp(w, h, pose), M.viewport(0, h, w, 0), rf.Framebuf(c, d),
+                      rf.Context(clear=True))
+        else:
+            rf.render(tr, (p, at), gour, S.spheres_mvp(w, h, pose), M.viewport(0, h, w, 0), rf.Framebuf(c, d),
+                      rf.Context(clear=True))
+    plain_fps = len(spec) / (time.perf_counter() - t0)
+    for _, me in enumerate(dmeshes):
         me.free()
     out["sweep_16_meshes"] = sweep
+    out["sweep_plain_sequential_frames_per_s"] = round(plain_fps, 1)
+    out["sweep_seconds_per_rate"] = a.serve_sweep_seconds
     return out
 
 
-def cpu_baseline(pos, uv, tris, tex, mvps, vp, W, H, seconds):
+def c3_run(rf, RB, a, W_steps):
+    """The batched workload at its stated size: 64 frames of uneven
+    resolution, each a seeded mix of the full crates and icosphere scenes
+    (10-100 % of each) from random poses, in one HIP-graph launch."""
+    from retrofire_amd import scenes as S
+    cpos, cuv, ctris, tex = S.crates_world(1000)
+    ipos, iattr, itris, _, nt = S.icospheres_world(10000)
+    frames = S.batch_frames(64, ctris.shape[0], itris.shape[0], nt, seed=0)
+    crates = RB.Mesh(cpos, ctris, cuv)
+    icos = RB.Mesh(ipos, itris, iattr)
+    texr = RB.Texture(tex)
+    fbs, spec = [], []
+    for f in frames:
+        fb = RB.DeviceFramebuf(f["w"], f["h"], 0, True)
+        fbs.append(fb)
+        spec.append((fb, rf.Context(clear=True), [
+            RB.mesh_draw(crates, rf.shader.TexRepeatPot(tex), f["mvp_c"], f["vp"], n_tris=f["nc"], texture=texr),
+            RB.mesh_draw(icos, rf.shader.Gouraud(), f["mvp_i"], f["vp"], n_tris=f["ni"])]))
+    t0 = time.perf_counter()
+    b = RB.Batch(spec)
+    plan_s = time.perf_counter() - t0
+    b.run(sync=True)
+    stats = b.wait()
+    t0 = time.perf_counter()
+    for _ in range(W_steps):
+        b.run(sync=False)
+    b.wait()
+    el = time.perf_counter() - t0
+    prof = b.profile(iters=3)
+    alg = sum(f["w"] * f["h"] * 8 + tex.nbytes for f in frames)
+    out = {"frames_per_step": 64, "steps": W_steps, "ms_per_step": round(1e3 * el / W_steps, 3),
+           "frames_per_s": round(64 * W_steps / el, 1),
+           "mfrag_per_s": round(sum(s.frags_i for s in stats) * W_steps / el / 1e6, 1),
+           "triangles_per_step": int(sum(f["nc"] + f["ni"] for f in frames)),
+           "pixels_per_step": int(sum(f["w"] * f["h"] for f in frames)),
+           "frags_o_per_step": int(sum(s.frags_o for s in stats)),
+           "tile_w": b.tile_w, "chunk": b.chunk, "plan_s": round(plan_s, 2),
+           "stage_ms": {n: round(x, 4) for n, x in zip(["vertex", "setup", "scan", "bin", "binsort", "raster"], prof)},
+           "raster_roofline": {"alg_bytes_per_launch": alg, "launch_ms": round(prof[5], 4),
+                               "achieved_GBs": round(alg / (prof[5] * 1e-3) / 1e9, 1),
+                               "frac": round(alg / (prof[5] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    b.free()
+    for fb in fbs:
+        fb.free()
+    crates.free()
+    icos.free()
+    texr.free()
+    return out
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(seconds):
+    """The oracle (a single-threaded C++ restatement of the reference CPU
+    path, -O2, fp-contract off) timed per configuration on one host core:
+    the process is pinned to one core while it runs; each frame is
+    Frame::clear plus one render() call; median over the sample."""
     import ctypes as C
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    from retrofire_amd import scenes as S
     L = O.lib()
-    col = np.zeros((H, W, 4), np.uint8)
-    dep = np.zeros((H, W), np.float32)
-    c = O.OracleCtx(1, 1, 1, 1)
-    clear = np.array([0, 0, 0, 255], np.uint8)
-    st = np.zeros(8, np.uint64)
-    times = []
-    t_end = time.perf_counter() + seconds
-    i = 0
-    tr = np.ascontiguousarray(tris.reshape(-1), np.uint32)
-    ps = np.ascontiguousarray(pos.reshape(-1), np.float32)
-    us = np.ascontiguousarray(uv.reshape(-1), np.float32)
-    tx = np.ascontiguousarray(tex)
-    vpf = np.ascontiguousarray(vp.reshape(-1), np.float32)
-    while (time.perf_counter() < t_end and i < 1000) or i < 3:
-        m = np.ascontiguousarray(mvps[i % len(mvps)].reshape(-1), np.float32)
-        per = C.c_double()
-        s = L.oracle_time_frames(3, O._p(tr, C.c_uint32), tris.shape[0], O._p(ps, C.c_float), O._p(us, C.c_float),
-                                 pos.shape[0], O._p(m, C.c_float), O._p(vpf, C.c_float), None,
-                                 tx.ctypes.data_as(C.POINTER(C.c_uint8)), tex.shape[1], tex.shape[0],
-                                 tex.shape[1], col.ctypes.data_as(C.POINTER(C.c_uint8)), W, H, 0,
-                                 O._p(dep, C.c_float), C.byref(c), clear.ctypes.data_as(C.POINTER(C.c_uint8)),
-                                 float("inf"), 1, C.byref(per), O._p(st, C.c_uint64))
-        assert s >= 0
-        times.append(s)
-        i += 1
-    med = float(np.median(times))
-    return {"value": round(1.0 / med, 3), "unit": "frames/s", "cores": 1, "kind": "port",
-            "sample": f"{len(times)} frames of the same workload (clear + render), median frame time "
-                      f"{med * 1e3:.1f} ms, single thread, C++ restatement of render() (-O2, fp-contract off)"}
+    old_aff = os.sched_getaffinity(0)
+    core = min(old_aff)
+    os.sched_setaffinity(0, {core})
+    budget = {"sphere640": 0.15, "crates1080": 0.35, "icospheres1080": 0.5}
+    cfgs = {}
+    try:
+        for name, key in (("sphere640", "c0"), ("crates1080", "c1"), ("icospheres1080", "c2")):
+            sc = S.SCENES[name]()
+            col = np.zeros((sc.height, sc.width, 4), np.uint8)
+            dep = np.zeros((sc.height, sc.width), np.float32)
+            c = O.OracleCtx(1, 1, 1, 1)
+            clear = np.array([0, 0, 0, 255], np.uint8)
+            flat = np.array([255, 255, 255, 255], np.uint8)
+            tr = np.ascontiguousarray(sc.tris.reshape(-1), np.uint32)
+            ps = np.ascontiguousarray(sc.pos.reshape(-1), np.float32)
+            at = np.ascontiguousarray(sc.attr.reshape(-1), np.float32)
+            m = np.ascontiguousarray(sc.mvp.reshape(-1), np.float32)
+            vpf = np.ascontiguousarray(sc.viewport.reshape(-1), np.float32)
+            tx = np.ascontiguousarray(sc.tex if sc.tex is not None else np.zeros((1, 1, 4), np.uint8))
+            times, frags = [], 0
+            t_end = time.perf_counter() + seconds * budget[name]
+            while (time.perf_counter() < t_end and len(times) < 1000) or len(times) < 3:
+                st = np.zeros(8, np.uint64)
+                per = (C.c_double * 1)()
+                s = L.oracle_time_frames(sc.kind, O._p(tr, C.c_uint32), sc.tris.shape[0], O._p(ps, C.c_float),
+                                         O._p(at, C.c_float), sc.pos.shape[0], O._p(m, C.c_float),
+                                         O._p(vpf, C.c_float), flat.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                         tx.ctypes.data_as(C.POINTER(C.c_uint8)), tx.shape[1], tx.shape[0],
+                                         tx.shape[1], col.ctypes.data_as(C.POINTER(C.c_uint8)), sc.width, sc.height,
+                                         0, O._p(dep, C.c_float), C.byref(c),
+                                         clear.ctypes.data_as(C.POINTER(C.c_uint8)), float("inf"), 1, per,
+                                         O._p(st, C.c_uint64))
+                assert s >= 0
+                times.append(s)
+                frags = int(st[5])
+            med = float(np.median(times))
+            cfgs[key] = {"workload": name, "frames": len(times), "median_ms": round(med * 1e3, 3),
+                         "frames_per_s": round(1.0 / med, 3), "mfrag_per_s": round(frags / med / 1e6, 3),
+                         "frags_i_per_frame": frags}
+    finally:
+        os.sched_setaffinity(0, old_aff)
+    return {"value": cfgs["c1"]["frames_per_s"], "unit": "frames/s", "cores": 1, "kind": "port",
+            "host_cpu": host_cpu_model(), "host_cores": os.cpu_count(), "pinned_core": core,
+            "sample": "per configuration, Frame::clear + render() frames on one pinned host core, median frame "
+                      "time: sphere640, crates1080 (the value), icospheres1080; the C++ restatement of render() "
+                      "(-O2, fp-contract off), not the Rust crate",
+            "configs": cfgs}
 
 
 if __name__ == "__main__":

@@ -43,8 +43,12 @@ typedef int32_t rf_status;
 #define RF_E_UNSUPPORTED (-5) /* not a built-in shader kind / target layout, or
                                  depth_sort on the batched or serving path: the
                                  caller keeps the CPU path                          */
-#define RF_E_VIEWPORT (-6)    /* to_screen maps the NDC cube outside the target   */
-#define RF_E_NONFINITE (-7)   /* a screen-space vertex is NaN or infinite          */
+#define RF_E_VIEWPORT (-6)    /* to_screen puts a vertex more than 2^22 rows above
+                                 the target (rows above row 0 fold onto it, as
+                                 `self.y as usize` does, raster.rs:106, up to that) */
+#define RF_E_NONFINITE (-7)   /* a screen-space vertex coordinate is infinite (a NaN
+                                 one, from a w = 0 vertex, is drawn as the
+                                 reference draws it: empty spans)                 */
 #define RF_E_INTERNAL (-8)    /* internal capacity or binning check failed         */
 #define RF_E_BUDGET (-9)      /* request larger than the serving memory budget     */
 

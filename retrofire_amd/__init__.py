@@ -199,9 +199,23 @@ def _fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+def _rows(a: np.ndarray) -> bool:
+    """Rows of packed elements at a positive row stride: a whole buffer or a
+    Slice2/MutSlice2 view of one (buf.rs:570-580), e.g. a numpy slice
+    `parent[y0:y1, x0:x1]` or a pitched surface."""
+    if a.ndim == 2:
+        px = a.itemsize
+    elif a.ndim == 3 and a.strides[2] == a.itemsize:
+        px = a.itemsize * a.shape[2]
+    else:
+        return False
+    return a.strides[1] == px and a.strides[0] >= px * a.shape[1] and a.strides[0] % a.itemsize == 0
+
+
 def make_target(target) -> _ffi.RfTarget:
     col = target.color
-    assert col.flags["C_CONTIGUOUS"]
+    if not _rows(col):
+        raise ValueError("colour buffer rows must hold packed pixels at a positive row stride")
     t = _ffi.RfTarget()
     t.color = col.ctypes.data
     t.height, t.width = col.shape[0], col.shape[1]
@@ -209,7 +223,8 @@ def make_target(target) -> _ffi.RfTarget:
     t.fmt = target.fmt
     if target.depth is not None:
         d = target.depth
-        assert d.dtype == np.float32 and d.flags["C_CONTIGUOUS"] and d.shape[:2] == col.shape[:2]
+        if not (d.dtype == np.float32 and d.ndim == 2 and _rows(d) and d.shape[:2] == col.shape[:2]):
+            raise ValueError("depth buffer must be float32 rows of the colour buffer's size")
         t.depth = _fptr(d)
         t.depth_stride = d.strides[0] // 4
     return t
@@ -260,6 +275,10 @@ def render(prims, verts, shd, uniform, to_screen, target, ctx: Context, device: 
     `uniform` is the mvp matrix of the built-in vertex shader; `to_screen` the
     viewport matrix. The target's arrays are updated in place; ctx.stats
     accumulates the call's statistics."""
+    _render(prims, verts, shd, uniform, to_screen, target, ctx, device, "rf_render")
+
+
+def _render(prims, verts, shd, uniform, to_screen, target, ctx, device, entry):
     if not isinstance(shd, (shader.Flat, shader.Gouraud, shader.TexClamp, shader.TexRepeatPot, shader.TexOnce)):
         raise RenderError(-5)
     dev = device or default_device()
@@ -268,5 +287,5 @@ def render(prims, verts, shd, uniform, to_screen, target, ctx: Context, device: 
     t = make_target(target)
     c = ctx.to_c()
     st = _ffi.RfStats()
-    check(_ffi.lib().rf_render(dev.h, C.byref(d), C.byref(t), C.byref(c), C.byref(st)))
+    check(getattr(_ffi.lib(), entry)(dev.h, C.byref(d), C.byref(t), C.byref(c), C.byref(st)))
     ctx.stats.add(st)

@@ -96,62 +96,73 @@ void set_grid(FrameDesc& f, uint32_t tile_w, uint32_t& tile_base) {
     tile_base += f.ntx * f.nty;
 }
 
-// Scratch sized for a batch; fills the scratch pointers of `B`.
-hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
-    hipError_t e;
+// The scratch arrays of a batch in block order: (bytes, where B keeps the
+// pointer). The misc words come first, so a block's first bytes are the
+// counters every launch reads back.
+template <class Fn>
+static void scratch_arrays(BatchDesc& B, Fn&& fn) {
     const uint64_t nv = std::max<uint64_t>(B.n_verts, 1), nt = std::max<uint64_t>(B.n_tris, 1);
-    if ((e = cv.ensure(nv * sizeof(float4), counter))) return e;
-    if ((e = ca.ensure(nv * sizeof(float4), counter))) return e;
-    if ((e = sv.ensure(nv * 6 * sizeof(float), counter))) return e;
-    if ((e = hdr.ensure(nt * sizeof(uint32_t), counter))) return e;
-    if ((e = scr.ensure(nt * sizeof(ScrTri), counter))) return e;
-    if ((e = extra.ensure(std::max<uint64_t>(B.extra_cap, 1) * sizeof(ScrTri), counter))) return e;
-    if ((e = rect.ensure(nt * 2 * sizeof(uint32_t), counter))) return e;
-    if ((e = group_draw.ensure((nt / 64 + 1) * sizeof(uint32_t), counter))) return e;
-    if ((e = vblock_draw.ensure((nv / 256 + 1) * sizeof(uint32_t), counter))) return e;
-    const uint64_t ntl = std::max<uint64_t>(B.n_tiles, 1);
-    if ((e = run_count.ensure(ntl * sizeof(uint32_t), counter))) return e;
-    if ((e = run_off.ensure((ntl + 1) * sizeof(uint32_t), counter))) return e;
-    if ((e = run_cursor.ensure(ntl * sizeof(uint32_t), counter))) return e;
-    if ((e = scan_blk.ensure(2 * (ntl / 1024 + 2) * sizeof(uint32_t), counter))) return e;
-    if ((e = tile_list.ensure(ntl * sizeof(uint32_t), counter))) return e;
-    if ((e = tile_frame.ensure(ntl * sizeof(uint32_t), counter))) return e;
-    if ((e = runs.ensure(std::max<uint64_t>(B.runs_cap, 1) * sizeof(Run), counter))) return e;
-    if ((e = misc.ensure(256, counter))) return e;
-    if ((e = stats.ensure(std::max<uint64_t>(B.n_frames, 1) * sizeof(FrameStats), counter))) return e;
-    if ((e = draw_vis.ensure(std::max<uint64_t>(B.n_draws, 1) * sizeof(uint32_t), counter))) return e;
-    if ((e = prims_part.ensure(std::max<uint64_t>(B.n_frames, 1) * PRIM_PARTS * PRIM_PART_STRIDE * 8, counter)))
-        return e;
-    B.cv = (float4*)cv.p;
-    B.ca = (float4*)ca.p;
-    B.sv = (float*)sv.p;
-    B.hdr = (uint32_t*)hdr.p;
-    B.scr = (ScrTri*)scr.p;
-    B.extra = (ScrTri*)extra.p;
-    B.rect = (uint32_t*)rect.p;
-    B.group_draw = (uint32_t*)group_draw.p;
-    B.vblock_draw = (uint32_t*)vblock_draw.p;
-    B.run_count = (uint32_t*)run_count.p;
-    B.run_off = (uint32_t*)run_off.p;
-    B.run_cursor = (uint32_t*)run_cursor.p;
-    B.scan_blk = (uint32_t*)scan_blk.p;
-    B.tile_frame = (uint32_t*)tile_frame.p;
-    B.tile_list = (uint32_t*)tile_list.p;
-    B.busy_count = (uint32_t*)misc.p + 8;
-    B.runs = (Run*)runs.p;
-    B.extra_count = (uint32_t*)misc.p;
-    B.tile_counter = (uint32_t*)misc.p + 16;  // TILE_PARTS + 1 words (busy partitions, empty tiles)
-    B.err = (uint32_t*)misc.p + 2;
-    B.stats = (FrameStats*)stats.p;
-    B.draw_vis = (uint32_t*)draw_vis.p;
-    B.prims_part = (unsigned long long*)prims_part.p;
+    const uint64_t ntl = std::max<uint64_t>(B.n_tiles, 1), nf = std::max<uint64_t>(B.n_frames, 1);
+    void* misc = nullptr;
+    fn(MISC_BYTES, &misc);
+    fn(nv * sizeof(float4), (void**)&B.cv);
+    fn(nv * sizeof(float4), (void**)&B.ca);
+    fn(nv * 6 * sizeof(float), (void**)&B.sv);
+    fn(nt * sizeof(uint32_t), (void**)&B.hdr);
+    fn(nt * sizeof(ScrTri), (void**)&B.scr);
+    fn(std::max<uint64_t>(B.extra_cap, 1) * sizeof(ScrTri), (void**)&B.extra);
+    fn(nt * 2 * sizeof(uint32_t), (void**)&B.rect);
+    fn((nt / 64 + 1) * sizeof(uint32_t), (void**)&B.group_draw);
+    fn((nv / 256 + 1) * sizeof(uint32_t), (void**)&B.vblock_draw);
+    fn(ntl * sizeof(uint32_t), (void**)&B.run_count);
+    fn((ntl + 1) * sizeof(uint32_t), (void**)&B.run_off);
+    fn(ntl * sizeof(uint32_t), (void**)&B.run_cursor);
+    fn(2 * (ntl / 1024 + 2) * sizeof(uint32_t), (void**)&B.scan_blk);
+    fn(ntl * sizeof(uint32_t), (void**)&B.tile_list);
+    fn(ntl * sizeof(uint32_t), (void**)&B.tile_frame);
+    fn(std::max<uint64_t>(B.runs_cap, 1) * sizeof(Run), (void**)&B.runs);
+    fn(nf * sizeof(FrameStats), (void**)&B.stats);
+    fn(std::max<uint64_t>(B.n_draws, 1) * sizeof(uint32_t), (void**)&B.draw_vis);
+    fn(nf * PRIM_PARTS * PRIM_PART_STRIDE * 8, (void**)&B.prims_part);
+    uint32_t* m = (uint32_t*)misc;
+    B.extra_count = m;
+    B.err = m ? m + 2 : nullptr;
+    B.busy_count = m ? m + 8 : nullptr;
+    B.tile_counter = m ? m + 16 : nullptr;  // TILE_PARTS + 1 words (busy partitions, empty tiles)
+}
+
+uint64_t scratch_bytes(const BatchDesc& B0) {
+    BatchDesc B = B0;
+    uint64_t off = 0;
+    scratch_arrays(B, [&](uint64_t bytes, void** p) {
+        *p = nullptr;
+        off += (bytes + 255) & ~uint64_t(255);
+    });
+    return off;
+}
+
+void scratch_carve(BatchDesc& B, uint8_t* base) {
+    uint64_t off = 0;
+    scratch_arrays(B, [&](uint64_t bytes, void** p) {
+        *p = base + off;
+        off += (bytes + 255) & ~uint64_t(255);
+    });
+}
+
+hipError_t Scratch::prepare(BatchDesc& B, uint64_t* counter) {
+    const hipError_t e = mem.ensure(scratch_bytes(B), counter);
+    if (e != hipSuccess) return e;
+    scratch_carve(B, (uint8_t*)mem.p);
     return hipSuccess;
 }
 
-void Scratch::release(uint64_t* counter) {
-    for (DevBuf* b : {&cv, &ca, &sv, &hdr, &scr, &extra, &rect, &run_count, &run_off, &run_cursor,
-                      &scan_blk, &tile_frame, &tile_list, &group_draw, &vblock_draw, &runs, &misc, &stats, &draw_vis, &prims_part})
-        b->release(counter);
+uint32_t grow_extra_cap(uint32_t needed) {
+    const uint64_t c = (uint64_t)needed + needed / 4 + 1024;
+    return (uint32_t)std::min<uint64_t>(c, MAX_LAUNCH_PRIMS);
+}
+uint32_t grow_runs_cap(uint32_t needed) {
+    const uint64_t c = (uint64_t)needed + needed / 4 + 1024;
+    return (uint32_t)std::min<uint64_t>(c, 0xFFFFFF00ull);
 }
 
 uint32_t raster_grid(const Device& d, uint32_t n_tiles) {
@@ -165,30 +176,34 @@ uint32_t raster_grid(const Device& d, uint32_t n_tiles) {
 rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const std::vector<FrameStats*>&,
                     std::vector<FrameStats>& stats_out, uint32_t* err_out,
                     const std::function<hipError_t()>& restore) {
+    if (B.n_tris >= MAX_LAUNCH_PRIMS) return RF_E_UNSUPPORTED;
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (sc.prepare(B, &d.bytes) != hipSuccess) return RF_E_DEVICE;
         if (launch_pipeline(B, s, raster_grid(d, B.n_tiles)) != hipSuccess) return RF_E_DEVICE;
         uint32_t misc[3];
         uint32_t total_runs = 0;
-        if (hipMemcpyAsync(misc, sc.misc.p, sizeof(misc), hipMemcpyDeviceToHost, s) != hipSuccess)
+        if (hipMemcpyAsync(misc, misc_words(B), sizeof(misc), hipMemcpyDeviceToHost, s) != hipSuccess)
             return RF_E_DEVICE;
-        if (hipMemcpyAsync(&total_runs, (uint32_t*)sc.run_off.p + B.n_tiles, 4, hipMemcpyDeviceToHost, s) !=
+        if (hipMemcpyAsync(&total_runs, B.run_off + B.n_tiles, 4, hipMemcpyDeviceToHost, s) !=
             hipSuccess)
             return RF_E_DEVICE;
         stats_out.resize(B.n_frames);
         if (B.n_frames &&
-            hipMemcpyAsync(stats_out.data(), sc.stats.p, B.n_frames * sizeof(FrameStats),
+            hipMemcpyAsync(stats_out.data(), B.stats, B.n_frames * sizeof(FrameStats),
                            hipMemcpyDeviceToHost, s) != hipSuccess)
             return RF_E_DEVICE;
         if (hipStreamSynchronize(s) != hipSuccess) return RF_E_DEVICE;
         const uint32_t err = misc[2];
         bool again = false;
         if (err & ERRB_EXTRA) {
-            B.extra_cap = misc[0] + misc[0] / 4 + 1024;
+            // more clip fans than a launch can index: refused, not wrapped
+            if (B.extra_cap >= MAX_LAUNCH_PRIMS) return RF_E_UNSUPPORTED;
+            B.extra_cap = grow_extra_cap(misc[0]);
             again = true;
         }
         if (err & ERRB_RUNS) {
-            B.runs_cap = total_runs + total_runs / 4 + 1024;
+            if (B.runs_cap >= 0xFFFFFF00u) return RF_E_UNSUPPORTED;
+            B.runs_cap = grow_runs_cap(total_runs);
             again = true;
         }
         if (!again) {
@@ -198,6 +213,146 @@ rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const s
         if (restore && restore() != hipSuccess) return RF_E_DEVICE;
     }
     return RF_E_INTERNAL;
+}
+
+
+// ---- the plain path's pieces, shared by rf_render and rf_render_noalloc ----
+
+rf_status plain_check(const rf_draw* draw, const rf_target* target, const rf_ctx* ctx) {
+    rf_status st = check_ctx(ctx);
+    if (st) return st;
+    if (draw->kind > KIND_LAST) return RF_E_UNSUPPORTED;
+    if (draw->prim > PRIM_EDGE) return RF_E_ARG;
+    if ((st = check_tex(draw->kind, draw->tex, draw->tex_w, draw->tex_h, draw->tex_stride))) return st;
+    if (!target->color || target->width == 0 || target->height == 0 || target->fmt > PIX_LAST ||
+        target->width > MAX_DIM || target->height > MAX_DIM)
+        return RF_E_ARG;
+    const uint32_t bpp = bytes_per_pixel(target->fmt);
+    if (target->color_stride < target->width * bpp) return RF_E_ARG;
+    if (target->depth && target->depth_stride < target->width) return RF_E_ARG;
+    if ((draw->n_tris && (!draw->tris || !draw->pos)) || (draw->n_verts && !draw->pos)) return RF_E_ARG;
+    if (draw->n_verts > 0xFFFFFFFFull) return RF_E_ARG;
+    if (draw->n_tris >= MAX_LAUNCH_PRIMS) return RF_E_UNSUPPORTED;
+    if (kind_attrs(draw->kind) && draw->n_verts && !draw->attr) return RF_E_ARG;
+    return RF_OK;
+}
+
+// Only the bytes a Slice2 view owns are read or written on the host: rows of
+// width elements at the view's stride, (h - 1) * stride + w elements from the
+// first (buf.rs:570-580); the gaps between the rows belong to the parent
+// buffer (an atlas, a pitched SDL surface). A target whose rows are packed
+// (stride = row bytes) keeps that pitch on the device and moves as one
+// block; otherwise the device copy has its own packed pitch and the rows
+// move by a 2D copy.
+PlainLayout plain_layout(const rf_draw* draw, const rf_target* target) {
+    PlainLayout L;
+    const uint32_t ipp = draw->prim == PRIM_EDGE ? 2u : 3u;  // indices per primitive
+    const uint32_t bpp = bytes_per_pixel(target->fmt);
+    L.na = kind_attrs(draw->kind);
+    L.textured = draw->kind >= KIND_TEX_CLAMP;
+    L.pos_b = draw->n_verts * 12;
+    L.attr_b = draw->n_verts * L.na * 4;
+    L.tri_b = draw->n_tris * ipp * 4;
+    L.tex_b = L.textured ? ((size_t)(draw->tex_h - 1) * draw->tex_stride + draw->tex_w) * 4 : 0;
+    L.col_row = (size_t)target->width * bpp;
+    L.col_pitch = target->color_stride == L.col_row ? L.col_row : (L.col_row + 63) & ~size_t(63);
+    L.col_b = (size_t)target->height * L.col_pitch;
+    L.dep_row = (size_t)target->width * 4;
+    L.dep_pitch_f = target->depth_stride == target->width ? target->width : (target->width + 15) & ~15u;
+    L.dep_b = target->depth ? (size_t)target->height * L.dep_pitch_f * 4 : 0;
+    return L;
+}
+
+hipError_t plain_upload_inputs(const rf_draw* draw, const PlainLayout& L, const PlainBufs& P, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    if (L.pos_b) e = hipMemcpyAsync(P.pos, draw->pos, L.pos_b, hipMemcpyHostToDevice, s);
+    if (!e && L.attr_b) e = hipMemcpyAsync(P.attr, draw->attr, L.attr_b, hipMemcpyHostToDevice, s);
+    if (!e && L.tri_b) e = hipMemcpyAsync(P.tris, draw->tris, L.tri_b, hipMemcpyHostToDevice, s);
+    if (!e && L.tex_b) e = hipMemcpyAsync(P.tex, draw->tex, L.tex_b, hipMemcpyHostToDevice, s);
+    return e;
+}
+
+hipError_t plain_upload_target(const rf_target* target, const rf_ctx* ctx, const PlainLayout& L, const PlainBufs& P,
+                               hipStream_t s) {
+    hipError_t e = hipSuccess;
+    if (!ctx->clear_color)
+        e = hipMemcpy2DAsync(P.color, L.col_pitch, target->color, target->color_stride, L.col_row, target->height,
+                             hipMemcpyHostToDevice, s);
+    if (!e && target->depth && !ctx->clear_depth)
+        e = hipMemcpy2DAsync(P.depth, L.dep_pitch_f * 4, target->depth, (size_t)target->depth_stride * 4, L.dep_row,
+                             target->height, hipMemcpyHostToDevice, s);
+    return e;
+}
+
+hipError_t plain_download(const rf_target* target, const PlainLayout& L, const PlainBufs& P, hipStream_t s) {
+    hipError_t e = hipMemcpy2DAsync(target->color, target->color_stride, P.color, L.col_pitch, L.col_row,
+                                    target->height, hipMemcpyDeviceToHost, s);
+    if (!e && target->depth)
+        e = hipMemcpy2DAsync(target->depth, (size_t)target->depth_stride * 4, P.depth, L.dep_pitch_f * 4, L.dep_row,
+                             target->height, hipMemcpyDeviceToHost, s);
+    return e;
+}
+
+void plain_descs(const rf_draw* draw, const rf_target* target, const rf_ctx* ctx, const PlainLayout& L,
+                 const PlainBufs& P, uint32_t tile_w, uint32_t chunk, DrawDesc& dd, FrameDesc& fd, BatchDesc& B) {
+    std::memset(&dd, 0, sizeof(dd));
+    dd.prim = draw->prim;
+    dd.pos = (const float*)P.pos;
+    dd.attr = (const float*)P.attr;
+    dd.tris = (const uint32_t*)P.tris;
+    dd.tex = L.textured ? (const uint8_t*)P.tex : nullptr;
+    dd.n_verts = (uint32_t)draw->n_verts;
+    dd.n_tris = (uint32_t)draw->n_tris;
+    dd.kind = draw->kind;
+    dd.frame = 0;
+    dd.tex_w = draw->tex_w;
+    dd.tex_h = draw->tex_h;
+    dd.tex_stride = draw->tex_stride;
+    dd.flat_rgba = pack_pixel(PIX_RGBA8, draw->flat_color[0], draw->flat_color[1], draw->flat_color[2],
+                              draw->flat_color[3]);
+    dd.attr_stride = (uint32_t)L.na;
+    dd.vert_base = 0;
+    dd.tri_base = 0;
+    std::memcpy(dd.mvp, draw->mvp, 64);
+    std::memcpy(dd.vp, draw->viewport, 64);
+
+    std::memset(&fd, 0, sizeof(fd));
+    fd.color = (uint8_t*)P.color;
+    fd.depth = target->depth ? (float*)P.depth : nullptr;
+    fd.color_stride = L.col_pitch;
+    fd.depth_stride = L.dep_pitch_f;
+    fd.width = target->width;
+    fd.height = target->height;
+    fd.fmt = target->fmt;
+    fill_frame_ctx(fd, *ctx);
+    uint32_t tile_base = 0;
+    set_grid(fd, tile_w, tile_base);
+
+    std::memset(&B, 0, sizeof(B));
+    B.draws = (const DrawDesc*)P.desc;
+    B.frames = (const FrameDesc*)((uint8_t*)P.desc + 256 * ((sizeof(DrawDesc) + 255) / 256));
+    B.n_draws = 1;
+    B.n_frames = 1;
+    B.n_verts = draw->n_verts;
+    B.n_tris = draw->n_tris;
+    B.n_groups = (uint32_t)((draw->n_tris + WAVE_SIZE - 1) / WAVE_SIZE);
+    B.n_tiles = tile_base;
+    B.tile_w = tile_w;
+    B.chunk = chunk;
+    B.sort_tpw = 1;  // one frame: one tile per binsort wave costs little and spreads dense tiles
+    B.tile_h = TILE_ROWS;
+}
+
+void plain_stats(rf_stats* stats, const rf_draw* draw, const FrameStats& fs, double t0) {
+    if (!stats) return;
+    stats->calls += 1;
+    stats->prims_i += draw->n_tris;
+    stats->verts_i += draw->n_verts;
+    stats->prims_o += fs.prims_o;
+    stats->verts_o += fs.verts_o;
+    stats->frags_i += fs.frags_i;
+    stats->frags_o += fs.frags_o;
+    stats->seconds += now_s() - t0;
 }
 
 }  // namespace rf
@@ -214,8 +369,8 @@ const char* rf_status_str(rf_status s) {
         case RF_E_ARG: return "invalid argument";
         case RF_E_DEVICE: return "HIP device error";
         case RF_E_UNSUPPORTED: return "unsupported shader, target or context (use the CPU path)";
-        case RF_E_VIEWPORT: return "to_screen maps geometry above the target's first row";
-        case RF_E_NONFINITE: return "non-finite screen-space vertex";
+        case RF_E_VIEWPORT: return "to_screen maps geometry more than 2^22 rows above the target";
+        case RF_E_NONFINITE: return "infinite screen-space vertex coordinate";
         case RF_E_INTERNAL: return "internal capacity check failed";
         case RF_E_BUDGET: return "request exceeds the serving memory budget";
     }
@@ -283,104 +438,33 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     const double t0 = now_s();
     if (!dev || !draw || !target) return RF_E_ARG;
     Device& d = *reinterpret_cast<Device*>(dev);
-    rf_status st = check_ctx(ctx);
+    rf_status st = plain_check(draw, target, ctx);
     if (st) return st;
-    if (draw->kind > KIND_LAST) return RF_E_UNSUPPORTED;
-    if (draw->prim > PRIM_EDGE) return RF_E_ARG;
-    const uint32_t ipp = draw->prim == PRIM_EDGE ? 2u : 3u;  // indices per primitive
-    if ((st = check_tex(draw->kind, draw->tex, draw->tex_w, draw->tex_h, draw->tex_stride))) return st;
-    if (!target->color || target->width == 0 || target->height == 0 || target->fmt > PIX_LAST ||
-        target->width > MAX_DIM || target->height > MAX_DIM)
-        return RF_E_ARG;
-    const uint32_t bpp = bytes_per_pixel(target->fmt);
-    if (target->color_stride < target->width * bpp) return RF_E_ARG;
-    if (target->depth && target->depth_stride < target->width) return RF_E_ARG;
-    if ((draw->n_tris && (!draw->tris || !draw->pos)) || (draw->n_verts && !draw->pos)) return RF_E_ARG;
-    if (draw->n_tris > 0xFFFFFFFFull || draw->n_verts > 0xFFFFFFFFull) return RF_E_ARG;
-    const int na = kind_attrs(draw->kind);
-    if (na && draw->n_verts && !draw->attr) return RF_E_ARG;
     if (hipSetDevice(d.ordinal) != hipSuccess) return RF_E_DEVICE;
     hipStream_t s = d.stream;
+    const PlainLayout L = plain_layout(draw, target);
 
     // uploads
-    const size_t pos_b = draw->n_verts * 12, attr_b = draw->n_verts * na * 4, tri_b = draw->n_tris * ipp * 4;
-    const bool textured = draw->kind >= KIND_TEX_CLAMP;
-    const size_t tex_b = textured ? (size_t)draw->tex_h * draw->tex_stride * 4 : 0;
-    const size_t col_b = (size_t)target->height * target->color_stride;
-    const size_t dep_b = target->depth ? (size_t)target->height * target->depth_stride * 4 : 0;
-    if (d.up_pos.ensure(std::max<size_t>(pos_b, 4), &d.bytes) || d.up_attr.ensure(std::max<size_t>(attr_b, 4), &d.bytes) ||
-        d.up_tris.ensure(std::max<size_t>(tri_b, 4), &d.bytes) || d.up_tex.ensure(std::max<size_t>(tex_b, 4), &d.bytes) ||
-        d.up_color.ensure(col_b, &d.bytes) || d.up_depth.ensure(std::max<size_t>(dep_b, 4), &d.bytes) ||
-        d.up_desc.ensure(sizeof(DrawDesc) + sizeof(FrameDesc) + 256, &d.bytes))
+    if (d.up_pos.ensure(std::max<size_t>(L.pos_b, 4), &d.bytes) ||
+        d.up_attr.ensure(std::max<size_t>(L.attr_b, 4), &d.bytes) ||
+        d.up_tris.ensure(std::max<size_t>(L.tri_b, 4), &d.bytes) ||
+        d.up_tex.ensure(std::max<size_t>(L.tex_b, 4), &d.bytes) || d.up_color.ensure(L.col_b, &d.bytes) ||
+        d.up_depth.ensure(std::max<size_t>(L.dep_b, 4), &d.bytes) ||
+        d.up_desc.ensure(PLAIN_DESC_BYTES, &d.bytes))
         return RF_E_DEVICE;
-    if (pos_b && hipMemcpyAsync(d.up_pos.p, draw->pos, pos_b, hipMemcpyHostToDevice, s)) return RF_E_DEVICE;
-    if (attr_b && hipMemcpyAsync(d.up_attr.p, draw->attr, attr_b, hipMemcpyHostToDevice, s)) return RF_E_DEVICE;
-    if (tri_b && hipMemcpyAsync(d.up_tris.p, draw->tris, tri_b, hipMemcpyHostToDevice, s)) return RF_E_DEVICE;
-    if (tex_b && hipMemcpyAsync(d.up_tex.p, draw->tex, tex_b, hipMemcpyHostToDevice, s)) return RF_E_DEVICE;
-    const bool load_color = !ctx->clear_color, load_depth = target->depth && !ctx->clear_depth;
-    auto upload_target = [&]() -> hipError_t {
-        hipError_t e = hipSuccess;
-        if (load_color) e = hipMemcpyAsync(d.up_color.p, target->color, col_b, hipMemcpyHostToDevice, s);
-        if (!e && load_depth) e = hipMemcpyAsync(d.up_depth.p, target->depth, dep_b, hipMemcpyHostToDevice, s);
-        return e;
-    };
+    PlainBufs P{d.up_pos.p, d.up_attr.p, d.up_tris.p, d.up_tex.p, d.up_color.p, d.up_depth.p, d.up_desc.p};
+    if (plain_upload_inputs(draw, L, P, s)) return RF_E_DEVICE;
+    auto upload_target = [&]() -> hipError_t { return plain_upload_target(target, ctx, L, P, s); };
     if (upload_target()) return RF_E_DEVICE;
 
     DrawDesc dd;
-    std::memset(&dd, 0, sizeof(dd));
-    dd.prim = draw->prim;
-    dd.pos = (const float*)d.up_pos.p;
-    dd.attr = (const float*)d.up_attr.p;
-    dd.tris = (const uint32_t*)d.up_tris.p;
-    dd.tex = textured ? (const uint8_t*)d.up_tex.p : nullptr;
-    dd.n_verts = (uint32_t)draw->n_verts;
-    dd.n_tris = (uint32_t)draw->n_tris;
-    dd.kind = draw->kind;
-    dd.frame = 0;
-    dd.tex_w = draw->tex_w;
-    dd.tex_h = draw->tex_h;
-    dd.tex_stride = draw->tex_stride;
-    dd.flat_rgba = pack_pixel(PIX_RGBA8, draw->flat_color[0], draw->flat_color[1], draw->flat_color[2],
-                              draw->flat_color[3]);
-    dd.attr_stride = (uint32_t)na;
-    dd.vert_base = 0;
-    dd.tri_base = 0;
-    std::memcpy(dd.mvp, draw->mvp, 64);
-    std::memcpy(dd.vp, draw->viewport, 64);
-
     FrameDesc fd;
-    std::memset(&fd, 0, sizeof(fd));
-    fd.color = (uint8_t*)d.up_color.p;
-    fd.depth = target->depth ? (float*)d.up_depth.p : nullptr;
-    fd.color_stride = target->color_stride;
-    fd.depth_stride = target->depth_stride;
-    fd.width = target->width;
-    fd.height = target->height;
-    fd.fmt = target->fmt;
-    fill_frame_ctx(fd, *ctx);
-    const uint32_t tw = d.plain_tile_w;
-    uint32_t tile_base = 0;
-    set_grid(fd, tw, tile_base);
-
-    uint8_t* desc = (uint8_t*)d.up_desc.p;
-    if (hipMemcpyAsync(desc, &dd, sizeof(dd), hipMemcpyHostToDevice, s) ||
-        hipMemcpyAsync(desc + 256 * ((sizeof(dd) + 255) / 256), &fd, sizeof(fd), hipMemcpyHostToDevice, s))
-        return RF_E_DEVICE;
-
     BatchDesc B;
-    std::memset(&B, 0, sizeof(B));
-    B.draws = (const DrawDesc*)desc;
-    B.frames = (const FrameDesc*)(desc + 256 * ((sizeof(dd) + 255) / 256));
-    B.n_draws = 1;
-    B.n_frames = 1;
-    B.n_verts = draw->n_verts;
-    B.n_tris = draw->n_tris;
-    B.n_groups = (uint32_t)((draw->n_tris + WAVE_SIZE - 1) / WAVE_SIZE);
-    B.n_tiles = tile_base;
-    B.tile_w = tw;
-    B.chunk = d.plain_chunk;
-    B.sort_tpw = 1;  // one frame: one tile per binsort wave costs little and spreads dense tiles
-    B.tile_h = TILE_ROWS;
+    plain_descs(draw, target, ctx, L, P, d.plain_tile_w, d.plain_chunk, dd, fd, B);
+    uint8_t* desc = (uint8_t*)P.desc;
+    if (hipMemcpyAsync(desc, &dd, sizeof(dd), hipMemcpyHostToDevice, s) ||
+        hipMemcpyAsync((void*)B.frames, &fd, sizeof(fd), hipMemcpyHostToDevice, s))
+        return RF_E_DEVICE;
     B.extra_cap = std::max<uint32_t>(d.plain_extra_cap, 1024u);
     B.runs_cap = std::max<uint32_t>(d.plain_runs_cap, 4 * B.n_groups + B.n_tiles + 1024);
 
@@ -389,7 +473,6 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     // order: Render::depth is +inf for every edge (render.rs:67-70), and
     // sort_unstable_by leaves a slice of equal keys as it is.
     if (ctx->depth_sort && draw->prim == PRIM_TRI && draw->n_tris) {
-        if (draw->n_tris >= (1ull << 29)) return RF_E_ARG;  // triangle << 3 | fan must fit 32 bits
         uint32_t cap = std::max<uint32_t>(d.ds_cap, (uint32_t)std::min<uint64_t>(2 * draw->n_tris + 1024, 1u << 30));
         uint32_t n_f = 0;
         for (int attempt = 0;; ++attempt) {
@@ -403,17 +486,18 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
             A.fans = (float*)d.ds_fans.p;
             A.fan_keys = (unsigned long long*)d.ds_keys.p;
             A.fan_vals = (uint32_t*)d.ds_vals.p;
-            A.fan_count = (uint32_t*)d.plain.misc.p + 32;
+            A.fan_count = misc_words(A) + 32;
             uint32_t misc[33];
             if (launch_fans(A, s) ||
-                hipMemcpyAsync(misc, d.plain.misc.p, sizeof(misc), hipMemcpyDeviceToHost, s) ||
+                hipMemcpyAsync(misc, misc_words(A), sizeof(misc), hipMemcpyDeviceToHost, s) ||
                 hipStreamSynchronize(s))
                 return RF_E_DEVICE;
             const uint32_t e = misc[2];
             if (e & ERRB_INDEX) return RF_E_INDEX;
             if (e & ERRB_CLIP) return RF_E_INTERNAL;
             if (e & ERRB_EXTRA) {
-                cap = misc[32] + misc[32] / 4 + 1024;
+                if (cap >= (1u << 30)) return RF_E_UNSUPPORTED;
+                cap = (uint32_t)std::min<uint64_t>((uint64_t)misc[32] + misc[32] / 4 + 1024, 1u << 30);
                 continue;
             }
             n_f = misc[32];
@@ -453,20 +537,8 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     d.plain_runs_cap = std::max(d.plain_runs_cap, B.runs_cap);
     if ((st = map_err(err))) return st;
 
-    if (hipMemcpyAsync(target->color, d.up_color.p, col_b, hipMemcpyDeviceToHost, s)) return RF_E_DEVICE;
-    if (target->depth && hipMemcpyAsync(target->depth, d.up_depth.p, dep_b, hipMemcpyDeviceToHost, s))
-        return RF_E_DEVICE;
-    if (hipStreamSynchronize(s)) return RF_E_DEVICE;
-    if (stats) {
-        stats->calls += 1;
-        stats->prims_i += draw->n_tris;
-        stats->verts_i += draw->n_verts;
-        stats->prims_o += fs[0].prims_o;
-        stats->verts_o += fs[0].verts_o;
-        stats->frags_i += fs[0].frags_i;
-        stats->frags_o += fs[0].frags_o;
-        stats->seconds += now_s() - t0;
-    }
+    if (plain_download(target, L, P, s) || hipStreamSynchronize(s)) return RF_E_DEVICE;
+    plain_stats(stats, draw, fs[0], t0);
     return RF_OK;
 }
 

@@ -103,6 +103,7 @@ static rf_status build_descs(Batch& b, uint32_t tile_w, uint32_t chunk) {
             b.h_draws.push_back(dd);
         }
     }
+    if (tbase >= MAX_LAUNCH_PRIMS) return RF_E_UNSUPPORTED;
     const size_t db = b.h_draws.size() * sizeof(DrawDesc), fb = b.h_frames.size() * sizeof(FrameDesc);
     const size_t foff = (db + 255) & ~size_t(255);
     if (b.d_desc.ensure(foff + fb + 256, &d.bytes)) return RF_E_DEVICE;
@@ -290,9 +291,9 @@ rf_status rf_batch_run(rf_batch* bh, int sync) {
     if (hipSetDevice(b.dev->ordinal)) return RF_E_DEVICE;
     if (hipEventRecord(b.ev0, b.stream) || hipGraphLaunch(b.exec, b.stream) || hipEventRecord(b.ev1, b.stream))
         return RF_E_DEVICE;
-    if (hipMemcpyAsync(b.h_stats, b.sc.stats.p, b.B.n_frames * sizeof(FrameStats), hipMemcpyDeviceToHost,
+    if (hipMemcpyAsync(b.h_stats, b.B.stats, b.B.n_frames * sizeof(FrameStats), hipMemcpyDeviceToHost,
                        b.stream) ||
-        hipMemcpyAsync(b.h_misc, b.sc.misc.p, 16, hipMemcpyDeviceToHost, b.stream))
+        hipMemcpyAsync(b.h_misc, misc_words(b.B), 16, hipMemcpyDeviceToHost, b.stream))
         return RF_E_DEVICE;
     b.pending = true;
     if (sync) return rf_batch_wait(bh, nullptr);

@@ -15,6 +15,15 @@ namespace rf {
 
 constexpr uint32_t WAVE_SIZE = 64;
 constexpr uint32_t MAX_DIM = 16384;         // largest target side (span records pack x in 16 bits)
+// Primitives (triangles or edges) of one launch, and clip fans stored beyond
+// the first of each triangle: the rasterizer's fan queue packs a primitive
+// index as index << 3 | fan and the per-primitive header an extra-fan base as
+// base << 3 | count, both in 32 bits. Larger launches are refused
+// (RF_E_UNSUPPORTED), never wrapped.
+constexpr uint64_t MAX_LAUNCH_PRIMS = 1ull << 29;
+// Clip-fan and bin-run capacities grown by a rerun that needs more.
+uint32_t grow_extra_cap(uint32_t needed);
+uint32_t grow_runs_cap(uint32_t needed);
 constexpr uint32_t TILE_ROWS = TILE_HEIGHT;  // rasterizer tile height
 #ifndef RF_RASTER_WG_PER_CU
 #define RF_RASTER_WG_PER_CU (TILE_HEIGHT > 32 ? 4 : 8)
@@ -29,12 +38,21 @@ struct DevBuf {
     void release(uint64_t* counter);
 };
 
-// Per-pipeline scratch buffers.
+// Pipeline scratch of a batch: one block of device memory carved into the
+// arrays BatchDesc points to (the counters' words first, then per-vertex,
+// per-triangle, per-tile and per-frame arrays, each 256-byte aligned).
+// scratch_bytes gives the block's size for B's counts and capacities;
+// scratch_carve sets B's pointers into a block of at least that size.
+constexpr uint64_t MISC_BYTES = 256;  // counters: extra count, error word, busy count, tile counters, fan count
+uint64_t scratch_bytes(const BatchDesc& B);
+void scratch_carve(BatchDesc& B, uint8_t* base);
+inline uint32_t* misc_words(const BatchDesc& B) { return B.extra_count; }
+
+// A grow-only block holding a batch's scratch.
 struct Scratch {
-    DevBuf cv, ca, sv, hdr, scr, extra, rect, run_count, run_off, run_cursor, scan_blk, tile_frame, tile_list, group_draw, vblock_draw, runs, misc,
-        stats, draw_vis, prims_part;
+    DevBuf mem;
     hipError_t prepare(BatchDesc& B, uint64_t* counter);
-    void release(uint64_t* counter);
+    void release(uint64_t* counter) { mem.release(counter); }
 };
 
 struct Device {
@@ -50,6 +68,7 @@ struct Device {
     uint32_t plain_chunk = 8;    // rasterizer span chunk of rf_render and of batches with a forced tile width
     uint32_t plain_extra_cap = 0, plain_runs_cap = 0;
 };
+
 
 struct Mesh {
     DevBuf pos, attr, tris;
@@ -84,6 +103,31 @@ uint32_t raster_grid(const Device& d, uint32_t n_tiles);
 rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const std::vector<FrameStats*>&,
                     std::vector<FrameStats>& stats_out, uint32_t* err_out,
                     const std::function<hipError_t()>& restore);
+
+// The plain path (one render() call with host buffers): argument checks,
+// the device layout of its inputs and target, the host<->device copies, the
+// descriptors and the stats. rf_render keeps the buffers in grow-only device
+// allocations; rf_render_noalloc carves them from a fixed static array.
+struct PlainLayout {
+    int na = 0;          // attribute floats per vertex
+    bool textured = false;
+    size_t pos_b = 0, attr_b = 0, tri_b = 0, tex_b = 0;
+    size_t col_row = 0, col_pitch = 0, col_b = 0;    // colour: row bytes, device pitch, device bytes
+    size_t dep_row = 0, dep_pitch_f = 0, dep_b = 0;  // depth: row bytes, device pitch (floats), device bytes
+};
+struct PlainBufs {
+    void *pos, *attr, *tris, *tex, *color, *depth, *desc;
+};
+constexpr size_t PLAIN_DESC_BYTES = 256 * ((sizeof(DrawDesc) + 255) / 256) + 256 * ((sizeof(FrameDesc) + 255) / 256);
+rf_status plain_check(const rf_draw* draw, const rf_target* target, const rf_ctx* ctx);
+PlainLayout plain_layout(const rf_draw* draw, const rf_target* target);
+hipError_t plain_upload_inputs(const rf_draw* draw, const PlainLayout& L, const PlainBufs& P, hipStream_t s);
+hipError_t plain_upload_target(const rf_target* target, const rf_ctx* ctx, const PlainLayout& L, const PlainBufs& P,
+                               hipStream_t s);
+hipError_t plain_download(const rf_target* target, const PlainLayout& L, const PlainBufs& P, hipStream_t s);
+void plain_descs(const rf_draw* draw, const rf_target* target, const rf_ctx* ctx, const PlainLayout& L,
+                 const PlainBufs& P, uint32_t tile_w, uint32_t chunk, DrawDesc& dd, FrameDesc& fd, BatchDesc& B);
+void plain_stats(rf_stats* stats, const rf_draw* draw, const FrameStats& fs, double t0);
 
 // Depth sort (rf_kernels.hip, rf_sort.hip): clip fans and their keys for the
 // single draw of `B`; the sort of (key, fan) pairs; the sorted fans written

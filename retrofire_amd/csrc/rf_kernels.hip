@@ -41,6 +41,10 @@ namespace rf {
 constexpr int WAVE = 64;
 constexpr int TILE_H = (int)TILE_HEIGHT;  // tile rows
 constexpr int BIN_MARGIN = 2;  // extra pixels around a triangle's bounding box
+// Rows above the target (y < 0) that a triangle may fold onto row 0; more is
+// RF_E_VIEWPORT (the row counter y0 + k, a float in the reference, stays
+// exact below 2^22).
+constexpr int32_t FOLD_ROWS_MAX = 1 << 22;
 // Tuning constants (overridable at build time for sweeps).
 #ifndef RF_FF_MIN
 #define RF_FF_MIN 12
@@ -276,16 +280,20 @@ __device__ __forceinline__ void write_scr(ScrTri* dst, const float* s0, const fl
     dst->kind = kind;
 }
 
+// Screen-space extent of a primitive's vertices. fminf / fmaxf skip NaN
+// operands, so the extent is that of the vertices' numbers; `nan` records a
+// NaN x or y, `inf` an infinite one.
 struct RectAcc {
     float xmin, xmax, ymin, ymax;
-    bool finite;
+    bool nan, inf;
     __device__ void init() {
         xmin = ymin = __builtin_inff();
         xmax = ymax = -__builtin_inff();
-        finite = true;
+        nan = inf = false;
     }
     __device__ void add(const float* s) {
-        if (!(__builtin_isfinite(s[0]) && __builtin_isfinite(s[1]))) finite = false;
+        nan = nan || s[0] != s[0] || s[1] != s[1];
+        inf = inf || __builtin_isinf(s[0]) || __builtin_isinf(s[1]);
         xmin = fminf(xmin, s[0]);
         xmax = fmaxf(xmax, s[0]);
         ymin = fminf(ymin, s[1]);
@@ -297,13 +305,21 @@ struct RectAcc {
 // floor(top_y + 0.5) .. floor(bot_y + 0.5) (scan, raster.rs:286-300). Columns
 // are the vertex x range widened by BIN_MARGIN pixels (stepped edges may drift
 // by rounding; the rasterizer checks every span stays inside).
+//
+// A triangle with a NaN vertex (the screen position of a w = 0 clip vertex,
+// 0/0) draws no pixel, but its rows whose left edge is NaN are spans
+// `0..x1` (`x0 as usize` of NaN is 0, raster.rs:104-107) that count in
+// frags.i: its columns start at 0.
 __device__ __forceinline__ bool tile_rect(const RectAcc& r, const FrameDesc& F, uint32_t tw,
                                           uint32_t* out) {
     const float r0 = floorf(r.ymin + 0.5f), r1 = floorf(r.ymax + 0.5f);
     if (!(r1 > r0)) return false;
     int64_t row0 = (int64_t)r0, row1 = (int64_t)r1 - 1;
-    int64_t c0 = (int64_t)floorf(r.xmin) - BIN_MARGIN, c1 = (int64_t)floorf(r.xmax) + BIN_MARGIN;
+    const float xmin = r.nan ? 0.0f : r.xmin, xmax = r.xmax >= xmin ? r.xmax : xmin;
+    int64_t c0 = (int64_t)floorf(xmin) - BIN_MARGIN, c1 = (int64_t)floorf(xmax) + BIN_MARGIN;
+    // rows above the target fold onto row 0 (`self.y as usize`, raster.rs:106)
     if (row0 < 0) row0 = 0;
+    if (row1 < 0) row1 = 0;
     if (c0 < 0) c0 = 0;
     if (row1 >= (int64_t)F.height) row1 = (int64_t)F.height - 1;
     if (c1 >= (int64_t)F.width) c1 = (int64_t)F.width - 1;
@@ -579,12 +595,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                 }
             }
             if (nf > 0) {
-                if (!ra.finite) {
+                if (ra.inf) {
+                    // an infinite screen coordinate: the reference walks up
+                    // to 2^32 scanlines for it; reported instead
                     atomicOr(B.err, ERRB_NONFINITE);
                     nf = 0;
-                } else if (ra.ymin < -0.5f) {
-                    // rows above -0.5 would map several scanlines onto row 0
-                    // through the saturating `as usize` (raster.rs:106)
+                } else if (ra.ymin < -(float)FOLD_ROWS_MAX) {
+                    // rows above the target fold onto row 0 through the
+                    // saturating `self.y as usize` (raster.rs:106), each a
+                    // span of its own; beyond this many the float row
+                    // counter of the reference loses its halves
                     atomicOr(B.err, ERRB_NEGY);
                     nf = 0;
                 } else {
@@ -924,11 +944,17 @@ __device__ __forceinline__ void swap6(float* a, float* b) {
         b[k] = t;
     }
 }
+// The total_cmp order key of a screen y. A NaN y (0/0, the screen position
+// of a w = 0 clip vertex) is keyed as x86-64's default NaN, which has its
+// sign bit set and so sorts before -inf: that is the NaN the reference's
+// arithmetic produces there (the GPU's own NaN has no defined sign).
+__device__ __forceinline__ int32_t y_key(float y) { return total_key(y == y ? y : u2f(0xFFC00000u)); }
+
 __device__ __forceinline__ void tri_setup(Slot& S, float (&va)[6], float (&vb)[6], float (&vc)[6], int na) {
     // stable sort by total_cmp of y (slice::sort_by is stable)
-    if (total_key(vb[1]) < total_key(va[1])) swap6(va, vb);
-    if (total_key(vc[1]) < total_key(vb[1])) swap6(vb, vc);
-    if (total_key(vb[1]) < total_key(va[1])) swap6(va, vb);
+    if (y_key(vb[1]) < y_key(va[1])) swap6(va, vb);
+    if (y_key(vc[1]) < y_key(vb[1])) swap6(vb, vc);
+    if (y_key(vb[1]) < y_key(va[1])) swap6(va, vb);
     const float* top = va;
     const float* mid0 = vb;
     const float* bot = vc;
@@ -1132,10 +1158,11 @@ struct SlotP {
     float dv[2][4];        // d(z, a0, a1, a2)/dx of the upper and lower trapezoid
     uint32_t kind_tag;     // shader kind | (draw - the tile's draw base) << 16
     uint32_t p0, p1, p2;   // flat: packed colour; textured: width, height, row stride (texels)
-    uint32_t ra_rec;       // first tile row with an edge record | that record << 16
+    uint32_t rec;          // the triangle's first edge record in the batch
     int32_t ysplit;        // first row of the lower trapezoid (absolute)
     uint32_t rxb;          // pixel columns of the binned tiles, first | end << 16 (margin check)
-    uint32_t pad;
+    int32_t ra;            // row of that record, relative to the tile's first row (negative:
+                           // a row above the target, folded onto row 0)
 };
 // Edge record of one (triangle, tile row): the left and right edge state
 // after the serial row steps of ScanlineIter::next (raster.rs:84-113).
@@ -1424,6 +1451,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
         unsigned long long fi_acc = 0, fo_acc = 0;
         uint32_t errs = 0;
         uint32_t fq_head = 0, fq_tail = 0;  // ring indices (wave-uniform)
+        uint32_t head_skip = 0;             // rows of the head triangle already taken (wave-uniform)
         uint32_t dbase = 0;                 // draw of the tile's first triangle
         bool dbase_set = false;
 
@@ -1536,20 +1564,32 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                         const uint32_t rr0 = B.rect[2 * gg], rr1 = B.rect[2 * gg + 1];
                         rx0b = (rr0 & 0xFFFF) * TW;
                         rx1b = ((rr1 & 0xFFFF) + 1) * TW;
+                        // rows of the triangle in this tile; in the first tile
+                        // row, the rows above the target too: `self.y as
+                        // usize` folds each onto row 0 as a span of its own
+                        // (raster.rs:106), in row order before row 0's own
                         ra = S.ylo - (int32_t)y0;
                         rb = S.yhi - (int32_t)y0;
-                        ra = ra < 0 ? 0 : ra;
+                        if (y0 > 0 || ra > 0) ra = ra < 0 ? 0 : ra;
                         rb = rb > (int32_t)nrows ? (int32_t)nrows : rb;
+                        if (lane == 0) ra += (int32_t)head_skip;  // rows an earlier batch took
                         rows_in = rb > ra ? (uint32_t)(rb - ra) : 0u;
                     }
                 }
+                // A triangle with more rows here than a batch holds (only
+                // with folded rows: otherwise at most TILE_H) goes in parts
+                // of REC_CAP rows, as the sole triangle of its batches.
+                const bool split = lane == 0 && rows_in > (uint32_t)REC_CAP;
+                if (split) rows_in = REC_CAP;
+                const bool head_split = __ballot(split) != 0ull;
                 uint32_t rec_tot;
                 const uint32_t rec_base = wave_excl_scan(rows_in, &rec_tot);
                 // take the leading triangles whose records fit (lane 0 always
-                // fits: it has at most TILE_H rows)
+                // fits: it has at most REC_CAP rows)
                 const bool fits = rec_base + rows_in <= (uint32_t)REC_CAP;
-                const unsigned long long nf_mask = ~__ballot(act && fits);
-                const uint32_t nuse = nf_mask == 0ull ? (uint32_t)WAVE : (uint32_t)(__ffsll((long long)nf_mask) - 1);
+                const unsigned long long nf_mask = ~__ballot(act && fits && !head_split);
+                const uint32_t nuse = head_split ? 1u
+                                    : nf_mask == 0ull ? (uint32_t)WAVE : (uint32_t)(__ffsll((long long)nf_mask) - 1);
                 const bool use = lane < nuse;
                 const uint32_t nrec_new = __shfl(rec_base + rows_in, nuse - 1, WAVE);
                 // draw tags are relative to the tile's first draw (draws only grow
@@ -1595,7 +1635,8 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                     }
                     P.p1 = D.tex_h;
                     P.p2 = D.tex_stride;
-                    P.ra_rec = (uint32_t)ra | (rec_base << 16);
+                    P.rec = rec_base;
+                    P.ra = ra;
                     P.ysplit = S.t[0].n ? (int32_t)S.t[0].y0i + (int32_t)S.t[0].n : -0x7FFFFFFF;
                     P.rxb = rx0b | (rx1b << 16);
                     if (rows_in) s_own[rec_base] = (int8_t)lane;
@@ -1619,9 +1660,10 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                         const Trap& T = S.t[h];
                         if (T.n == 0) continue;
                         const int32_t ty0 = (int32_t)T.y0i;  // absolute first row
+                        const int32_t rend = ra + (int32_t)rows_in;  // rows recorded in this batch
                         int32_t a = ty0 > (int32_t)y0 + ra ? ty0 : (int32_t)y0 + ra;
                         int32_t b = ty0 + (int32_t)T.n;
-                        if (b > (int32_t)y0 + rb) b = (int32_t)y0 + rb;
+                        if (b > (int32_t)y0 + rend) b = (int32_t)y0 + rend;
                         if (a >= b) continue;
                         float ev[6] = {T.lx, T.rx, T.lz, T.la[0], T.la[1], T.la[2]};
                         const float ed[6] = {T.dlx, T.drx, T.dlz, T.dla[0], T.dla[1], T.dla[2]};
@@ -1645,7 +1687,12 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                 nrec = nrec_new;
                 next = 0;
                 carry = -1;
-                fq_head += nuse;
+                if (head_split) {
+                    head_skip += REC_CAP;  // the same triangle heads the next batch
+                } else {
+                    fq_head += nuse;
+                    head_skip = 0;
+                }
             }
         rounds:
             const unsigned long long need = __ballot(!have);
@@ -1686,8 +1733,9 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                             itag = P.kind_tag >> 16;
                         }
                     } else {
-                    const int32_t r = (int32_t)(P.ra_rec & 0xFFFFu) + (int32_t)(k - (P.ra_rec >> 16));
-                    const int32_t y = (int32_t)y0 + r;
+                    const int32_t r = P.ra + (int32_t)(k - P.rec);
+                    const int32_t y = (int32_t)y0 + r;  // absolute row; negative: folded onto row 0
+                    const uint32_t ty = y > 0 ? (uint32_t)y : 0u;
                     const int h = y < P.ysplit ? 0 : 1;
                     ikind = P.kind_tag & 0xFFu;
                     dz = P.dv[h][0];
@@ -1708,7 +1756,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                     uint32_t xe = cvt_u32(rx1);
                     if (xe < xs32) xe = xs32;
                     int32_t jj0 = 0, jj1 = 0;
-                    if (xe > F.width || (uint32_t)y >= F.height) {
+                    if (xe > F.width || ty >= F.height) {
                         errs |= ERRB_BOUNDS;
                     } else {
                         if (xe > xs32 && (xs32 < (P.rxb & 0xFFFFu) || xe > (P.rxb >> 16))) errs |= ERRB_MARGIN;
@@ -1732,7 +1780,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                         c2 = vv[3];
                         have = true;
                         key = key_base + k;
-                        row = (uint32_t)r;
+                        row = ty - y0;
                         jc = jj0;
                         j1 = jj1;
                         ip0 = P.p0;

@@ -4,14 +4,26 @@
 // draws over resident meshes) are admitted in arrival order while their
 // device memory fits the budget; whatever is admitted when a stream is free
 // goes out as one pipeline launch (continuous batching). Up to `n_streams`
-// batches are on the device at once, so frames that arrive while others are
+// launches are on the device at once, so frames that arrive while others are
 // rendering overlap them on the GPU. Each stream has a waiter thread that
-// blocks on an event recorded behind its batch; results carry submit/start/done
-// times.
+// blocks on an event recorded behind its launch; results carry
+// submit/start/done times.
+//
+// Memory: the budget is one device allocation made when the loop opens (the
+// arena). A launch takes one block of it for everything it touches on the
+// device -- its draw and frame descriptors, the pipeline scratch and the
+// targets of its frames -- and gives it back once its results are read back.
+// So the budget bounds the loop's device memory, and the steady-state loop
+// makes no device allocation (no hipMalloc or hipFree, which would
+// synchronise the device under the other streams' work). The scratch of a
+// launch is sized from its counts and from clip-fan and bin-run ratios
+// learned from earlier launches; a launch that needs more is rerun with a
+// larger block.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -19,21 +31,81 @@
 #include <mutex>
 #include <new>
 #include <thread>
-#include <tuple>
 #include <vector>
 
 #include "../../include/retrofire_amd.h"
 #include "rf_internal.hpp"
 
+#ifndef RF_SERVE_CHUNK
+// 6-pixel span chunks: served launches mix meshes of small triangles, where
+// they shorten the latency, with larger ones, where they cost nothing
+// measurable
+#define RF_SERVE_CHUNK 6
+#endif
+#ifndef RF_SERVE_TILE_W
+#define RF_SERVE_TILE_W 32
+#endif
+
 namespace rf {
 
 namespace {
+
+constexpr uint32_t SERVE_TILE_W = RF_SERVE_TILE_W;  // rasterizer tile width of served launches
+constexpr uint64_t ALIGN = 256;
+inline uint64_t align_up(uint64_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
+
+// First-fit allocator over the arena's byte range; free ranges coalesce.
+// Used by the worker thread only.
+struct Arena {
+    uint8_t* base = nullptr;
+    uint64_t size = 0, used = 0;
+    std::map<uint64_t, uint64_t> free_;  // offset -> bytes
+    void init(uint8_t* b, uint64_t n) {
+        base = b;
+        size = n;
+        used = 0;
+        free_.clear();
+        if (n) free_[0] = n;
+    }
+    uint64_t largest() const {
+        uint64_t m = 0;
+        for (const auto& kv : free_) m = std::max(m, kv.second);
+        return m;
+    }
+    bool alloc(uint64_t bytes, uint64_t& off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            off = it->first;
+            const uint64_t rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[off + bytes] = rest;
+            used += bytes;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t bytes) {
+        used -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
 
 struct Req {
     rf_request r;
     std::vector<rf_mesh_draw> draws;
     double submit_s = 0;
-    uint64_t bytes = 0;       // budget charge
     uint64_t n_tris = 0, n_verts = 0;
 };
 
@@ -43,30 +115,41 @@ struct Done {
     size_t pix_bytes = 0;
 };
 
-using FbKey = std::tuple<uint32_t, uint32_t, uint32_t, int>;
+// Counts of a launch: what its block must hold.
+struct Plan {
+    uint64_t n_verts = 0, n_tris = 0, n_draws = 0, n_frames = 0, n_tiles = 0, targets = 0;
+    void add(const Req& q) {
+        n_verts += q.n_verts;
+        n_tris += q.n_tris;
+        n_draws += q.draws.size();
+        n_frames += 1;
+        n_tiles += (uint64_t)((q.r.width + SERVE_TILE_W - 1) / SERVE_TILE_W) * ((q.r.height + TILE_ROWS - 1) / TILE_ROWS);
+        const uint64_t px = (uint64_t)q.r.width * q.r.height;
+        targets += align_up(px * bytes_per_pixel(q.r.fmt)) + (q.r.with_depth ? align_up(px * 4) : 0);
+    }
+};
 
 struct Lane {
     hipStream_t stream = nullptr;
-    Scratch sc;
-    DevBuf desc;
     uint8_t* h_desc = nullptr;  // pinned staging for descriptors
     size_t h_desc_cap = 0;
     FrameStats* h_stats = nullptr;
     uint32_t* h_misc = nullptr;
     uint32_t stats_cap = 0;
-    uint32_t extra_cap = 4096, runs_cap = 1 << 16;
     bool busy = false;
-    // the batch in flight
+    bool wait_mem = false;   // rerun needs a larger block than is free now
+    // the launch in flight
     std::vector<Req> reqs;
-    std::vector<Framebuf*> fbs;
     std::vector<uint8_t*> pix;
     std::vector<size_t> pix_bytes;
+    uint64_t blk_off = 0, blk_bytes = 0;  // arena block (blk_bytes 0: none)
+    uint32_t extra_cap = 0, runs_cap = 0;
     BatchDesc B{};
     double start_s = 0;
     std::atomic<int> finished{0};
     double done_s = 0;
     int attempts = 0;
-    // completion: the worker records `ev` behind the batch and sets `launched`;
+    // completion: the worker records `ev` behind the launch and sets `launched`;
     // the lane's waiter thread blocks in hipEventSynchronize (holding no lock),
     // then sets `finished`
     hipEvent_t ev = nullptr;
@@ -79,9 +162,12 @@ struct Lane {
 
 struct Serve {
     Device* dev = nullptr;
-    uint64_t bytes = 0;  // device memory held by the serving loop (worker thread only)
-    uint64_t budget = 0, used = 0;
+    void* arena_mem = nullptr;
+    Arena arena;             // worker thread only
     uint32_t max_batch = 16;
+    // learned capacity ratios (worker thread only): clip fans beyond the
+    // first per triangle, bin runs per 64-triangle group
+    double extra_ratio = 1.0 / 16, runs_ratio = 8.0;
     std::vector<Lane*> lanes;
     std::thread worker;
     std::mutex m;
@@ -90,8 +176,7 @@ struct Serve {
     std::deque<Done> done;
     bool stop = false;
     bool stop_waiters = false;
-    std::mutex pm;  // guards the pools; never held across a HIP call
-    std::map<FbKey, std::vector<Framebuf*>> fb_pool;
+    std::mutex pm;  // guards the pinned pool; never held across a HIP call
     std::map<size_t, std::vector<uint8_t*>> pin_pool;
 };
 
@@ -114,37 +199,6 @@ static void waiter_main(Serve* sp, Lane* l) {
     }
 }
 
-static Framebuf* fb_get(Serve& s, uint32_t w, uint32_t h, uint32_t fmt, int depth) {
-    {
-        std::lock_guard<std::mutex> g(s.pm);
-        auto& v = s.fb_pool[FbKey(w, h, fmt, depth)];
-        if (!v.empty()) {
-            Framebuf* f = v.back();
-            v.pop_back();
-            return f;
-        }
-    }
-    Framebuf* f = new Framebuf();
-    f->w = w;
-    f->h = h;
-    f->fmt = fmt;
-    f->stride = w * bytes_per_pixel(fmt);
-    f->has_depth = depth != 0;
-    uint64_t bytes = 0;
-    if (f->color.ensure((size_t)h * f->stride, &bytes) || (depth && f->depth.ensure((size_t)w * h * 4, &bytes))) {
-        f->color.release(&bytes);
-        f->depth.release(&bytes);
-        delete f;
-        return nullptr;
-    }
-    return f;
-}
-static void fb_put(Serve& s, Framebuf* f) {
-    if (!f) return;
-    std::lock_guard<std::mutex> g(s.pm);
-    s.fb_pool[FbKey(f->w, f->h, f->fmt, f->has_depth)].push_back(f);
-}
-
 static uint8_t* pin_get(Serve& s, size_t bytes) {
     {
         std::lock_guard<std::mutex> g(s.pm);
@@ -165,19 +219,40 @@ static void pin_put(Serve& s, uint8_t* p, size_t bytes) {
     s.pin_pool[bytes].push_back(p);
 }
 
-// Device bytes a request holds while in flight: its target plus the pipeline
-// scratch its geometry needs.
-static uint64_t request_bytes(const Req& q) {
-    const uint64_t bpp = bytes_per_pixel(q.r.fmt);
-    const uint64_t px = (uint64_t)q.r.width * q.r.height;
-    const uint64_t target = px * (bpp + (q.r.with_depth ? 4 : 0));
-    const uint64_t geom = q.n_verts * 56 + q.n_tris * (sizeof(ScrTri) + 12) + (q.n_tris / 64 + 1) * 4 * sizeof(Run);
-    return target + geom;
+static uint32_t est_extra(const Serve& s, const Plan& p) {
+    const double e = std::ceil((double)p.n_tris * s.extra_ratio) + 1024.0;
+    return (uint32_t)std::min<double>(e, (double)MAX_LAUNCH_PRIMS);
+}
+static uint32_t est_runs(const Serve& s, const Plan& p) {
+    const double groups = std::ceil((double)p.n_tris / WAVE_SIZE);
+    const double r = std::ceil(groups * s.runs_ratio) + (double)p.n_tiles + 4096.0;
+    return (uint32_t)std::min<double>(r, 4.0e9);
+}
+
+// Bytes of the block of a launch with counts `p` and capacities (extra, runs):
+// descriptors, scratch and targets.
+static uint64_t block_bytes(const Plan& p, uint32_t extra, uint32_t runs) {
+    BatchDesc B;
+    std::memset(&B, 0, sizeof(B));
+    B.n_verts = p.n_verts;
+    B.n_tris = p.n_tris;
+    B.n_draws = (uint32_t)p.n_draws;
+    B.n_frames = (uint32_t)p.n_frames;
+    B.n_tiles = (uint32_t)std::min<uint64_t>(p.n_tiles, 0xFFFFFFFFull);
+    B.extra_cap = extra;
+    B.runs_cap = runs;
+    const uint64_t desc = align_up(p.n_draws * sizeof(DrawDesc)) + align_up(p.n_frames * sizeof(FrameDesc));
+    return desc + scratch_bytes(B) + p.targets;
+}
+
+static Plan plan_of(const std::vector<Req>& reqs) {
+    Plan p;
+    for (const Req& q : reqs) p.add(q);
+    return p;
 }
 
 static void finish_lane(Serve& s, Lane& l, rf_status st) {
-    // called with s.m held
-    std::vector<FrameStats> fs(l.reqs.size());
+    // called with s.m held, on the worker thread
     for (size_t i = 0; i < l.reqs.size(); ++i) {
         Req& q = l.reqs[i];
         Done d;
@@ -202,63 +277,112 @@ static void finish_lane(Serve& s, Lane& l, rf_status st) {
             d.res.stats.prims_i -= h.hidden_prims;
             d.res.stats.verts_i -= h.hidden_verts;
             d.res.stats.objs_o -= h.hidden_draws;
-            d.res.stats.prims_o = l.h_stats[i].prims_o;
-            d.res.stats.verts_o = l.h_stats[i].verts_o;
-            d.res.stats.frags_i = l.h_stats[i].frags_i;
-            d.res.stats.frags_o = l.h_stats[i].frags_o;
+            d.res.stats.prims_o = h.prims_o;
+            d.res.stats.verts_o = h.verts_o;
+            d.res.stats.frags_i = h.frags_i;
+            d.res.stats.frags_o = h.frags_o;
         }
         d.res.stats.seconds = l.done_s - l.start_s;
-        d.pixels = l.pix[i];
-        d.pix_bytes = l.pix_bytes[i];
+        if (i < l.pix.size()) {
+            d.pixels = l.pix[i];
+            d.pix_bytes = l.pix_bytes[i];
+        }
         s.done.push_back(d);
-        s.used -= q.bytes;
-        fb_put(s, l.fbs[i]);
     }
+    if (l.blk_bytes) s.arena.release(l.blk_off, l.blk_bytes);
+    l.blk_bytes = 0;
     l.reqs.clear();
-    l.fbs.clear();
     l.pix.clear();
     l.pix_bytes.clear();
     l.busy = false;
+    l.wait_mem = false;
     s.cv_done.notify_all();
 }
 
-// Builds descriptors for the lane's requests and enqueues the pipeline, the
-// readbacks and the completion event. Called by the worker without s.m.
+// Fails one request that can never run.
+static void reject(Serve& s, const Req& q, rf_status st) {
+    Done d;
+    std::memset(&d.res, 0, sizeof(d.res));
+    d.res.id = q.r.id;
+    d.res.status = st;
+    d.res.width = q.r.width;
+    d.res.height = q.r.height;
+    d.res.fmt = q.r.fmt;
+    d.res.submit_s = q.submit_s;
+    d.res.start_s = d.res.done_s = now_s();
+    s.done.push_back(d);
+    s.cv_done.notify_all();
+}
+
+// Sizes the lane's block for its requests at the current capacities and
+// takes it from the arena. Returns false (nothing taken) if no free range is
+// large enough now.
+static bool take_block(Serve& s, Lane& l) {
+    const Plan p = plan_of(l.reqs);
+    const uint64_t need = block_bytes(p, l.extra_cap, l.runs_cap);
+    uint64_t off = 0;
+    if (!s.arena.alloc(need, off)) return false;
+    l.blk_off = off;
+    l.blk_bytes = need;
+    return true;
+}
+
+// Lays the lane's block out (descriptors, scratch, targets), uploads the
+// descriptors and enqueues the pipeline, the readbacks and the completion
+// event. Called by the worker without s.m.
 static rf_status launch_lane(Serve& s, Lane& l) {
     Device& d = *s.dev;
+    uint8_t* blk = (uint8_t*)s.arena.base + l.blk_off;
+    const Plan p = plan_of(l.reqs);
+    const uint64_t db = align_up(p.n_draws * sizeof(DrawDesc)), fb = align_up(p.n_frames * sizeof(FrameDesc));
+    BatchDesc B;
+    std::memset(&B, 0, sizeof(B));
+    B.draws = (const DrawDesc*)blk;
+    B.frames = (const FrameDesc*)(blk + db);
+    B.n_draws = (uint32_t)p.n_draws;
+    B.n_frames = (uint32_t)p.n_frames;
+    B.n_verts = p.n_verts;
+    B.n_tris = p.n_tris;
+    B.n_groups = (uint32_t)((p.n_tris + WAVE_SIZE - 1) / WAVE_SIZE);
+    B.n_tiles = (uint32_t)p.n_tiles;
+    B.tile_w = SERVE_TILE_W;
+    B.tile_h = TILE_ROWS;
+    B.chunk = RF_SERVE_CHUNK;  // span chunk of served launches
+    B.sort_tpw = 1;            // few frames a launch: one tile per binsort wave
+    B.extra_cap = l.extra_cap;
+    B.runs_cap = l.runs_cap;
+    const uint64_t sb = scratch_bytes(B);
+    scratch_carve(B, blk + db + fb);
+    uint8_t* tgt = blk + db + fb + sb;
+
     std::vector<DrawDesc> dds;
     std::vector<FrameDesc> fds;
+    dds.reserve(p.n_draws);
+    fds.reserve(p.n_frames);
     uint64_t vbase = 0, tbase = 0;
-    uint32_t tile_base = 0;
-#ifndef RF_SERVE_CHUNK
-// 6-pixel span chunks: served launches mix meshes of small triangles, where
-// they shorten the latency, with larger ones, where they cost nothing
-// measurable
-#define RF_SERVE_CHUNK 6
-#endif
-#ifndef RF_SERVE_TILE_W
-#define RF_SERVE_TILE_W 32
-#endif
-    const uint32_t tile_w = RF_SERVE_TILE_W;  // rasterizer tile width of served launches
-    uint32_t any_cull = 0;
+    uint32_t tile_base = 0, any_cull = 0;
     for (size_t i = 0; i < l.reqs.size(); ++i) {
         Req& q = l.reqs[i];
-        Framebuf* fb = l.fbs[i];
-        FrameDesc fd;
-        std::memset(&fd, 0, sizeof(fd));
-        fd.color = (uint8_t*)fb->color.p;
-        fd.depth = fb->has_depth ? (float*)fb->depth.p : nullptr;
-        fd.color_stride = fb->stride;
-        fd.depth_stride = fb->w;
-        fd.width = fb->w;
-        fd.height = fb->h;
-        fd.fmt = fb->fmt;
+        const uint64_t px = (uint64_t)q.r.width * q.r.height;
+        FrameDesc f;
+        std::memset(&f, 0, sizeof(f));
+        f.color = tgt;
+        tgt += align_up(px * bytes_per_pixel(q.r.fmt));
+        if (q.r.with_depth) {
+            f.depth = (float*)tgt;
+            tgt += align_up(px * 4);
+        }
+        f.color_stride = (uint64_t)q.r.width * bytes_per_pixel(q.r.fmt);
+        f.depth_stride = q.r.width;
+        f.width = q.r.width;
+        f.height = q.r.height;
+        f.fmt = q.r.fmt;
         rf_ctx c = q.r.ctx;
         c.clear_color = 1;  // a served frame starts from Frame::clear
         c.clear_depth = 1;
-        fill_frame_ctx(fd, c);
-        set_grid(fd, tile_w, tile_base);
-        fds.push_back(fd);
+        fill_frame_ctx(f, c);
+        set_grid(f, SERVE_TILE_W, tile_base);
+        fds.push_back(f);
         for (const rf_mesh_draw& md : q.draws) {
             const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
             const Tex* t = reinterpret_cast<const Tex*>(md.tex);
@@ -292,161 +416,182 @@ static rf_status launch_lane(Serve& s, Lane& l) {
             dds.push_back(dd);
         }
     }
-    const size_t db = dds.size() * sizeof(DrawDesc), fb = fds.size() * sizeof(FrameDesc);
-    const size_t foff = (db + 255) & ~size_t(255);
-    const size_t total = foff + fb + 256;
-    if (l.desc.ensure(total, &s.bytes)) return RF_E_DEVICE;
+    B.any_cull = any_cull;
+    // pinned staging of the descriptors; it grows only when a launch needs
+    // more than any before (host memory, never the device's)
+    const size_t total = db + fb;
     if (total > l.h_desc_cap) {
         if (l.h_desc) (void)hipHostFree(l.h_desc);
         l.h_desc = nullptr;
+        l.h_desc_cap = 0;
         if (hipHostMalloc((void**)&l.h_desc, total * 2, 0)) return RF_E_DEVICE;
         l.h_desc_cap = total * 2;
     }
     if ((uint32_t)fds.size() > l.stats_cap) {
         if (l.h_stats) (void)hipHostFree(l.h_stats);
         l.h_stats = nullptr;
+        l.stats_cap = 0;
         if (hipHostMalloc((void**)&l.h_stats, fds.size() * 2 * sizeof(FrameStats), 0)) return RF_E_DEVICE;
         l.stats_cap = (uint32_t)fds.size() * 2;
     }
-    if (db) std::memcpy(l.h_desc, dds.data(), db);
-    std::memcpy(l.h_desc + foff, fds.data(), fb);
-    if (hipMemcpyAsync(l.desc.p, l.h_desc, total, hipMemcpyHostToDevice, l.stream)) return RF_E_DEVICE;
-    BatchDesc B;
-    std::memset(&B, 0, sizeof(B));
-    B.draws = (const DrawDesc*)l.desc.p;
-    B.frames = (const FrameDesc*)((uint8_t*)l.desc.p + foff);
-    B.n_draws = (uint32_t)dds.size();
-    B.n_frames = (uint32_t)fds.size();
-    B.n_verts = vbase;
-    B.n_tris = tbase;
-    B.n_groups = (uint32_t)((tbase + WAVE_SIZE - 1) / WAVE_SIZE);
-    B.any_cull = any_cull;
-    B.n_tiles = tile_base;
-    B.tile_w = tile_w;
-    B.tile_h = TILE_ROWS;
-    B.chunk = RF_SERVE_CHUNK;  // span chunk of served launches
-    B.sort_tpw = 1;            // few frames a launch: one tile per binsort wave
-    B.extra_cap = l.extra_cap;
-    B.runs_cap = std::max<uint32_t>(l.runs_cap, 8 * B.n_groups + B.n_tiles + 4096);
-    if (l.sc.prepare(B, &s.bytes)) return RF_E_DEVICE;
+    std::memcpy(l.h_desc, dds.data(), dds.size() * sizeof(DrawDesc));
+    std::memcpy(l.h_desc + db, fds.data(), fds.size() * sizeof(FrameDesc));
     l.B = B;
     l.start_s = now_s();
+    // Nothing is queued before this point; from here on a failure leaves
+    // work on the stream, which the caller drains before the block is reused.
+    if (hipMemcpyAsync(blk, l.h_desc, total, hipMemcpyHostToDevice, l.stream)) return RF_E_DEVICE;
     if (launch_pipeline(B, l.stream, raster_grid(d, B.n_tiles))) return RF_E_DEVICE;
-    if (hipMemcpyAsync(l.h_stats, l.sc.stats.p, fds.size() * sizeof(FrameStats), hipMemcpyDeviceToHost,
-                       l.stream) ||
-        hipMemcpyAsync(l.h_misc, l.sc.misc.p, 12, hipMemcpyDeviceToHost, l.stream))
-        return RF_E_DEVICE;
-    if (hipMemcpyAsync(l.h_misc + 4, (uint32_t*)l.sc.run_off.p + B.n_tiles, 4, hipMemcpyDeviceToHost, l.stream))
+    if (hipMemcpyAsync(l.h_stats, B.stats, fds.size() * sizeof(FrameStats), hipMemcpyDeviceToHost, l.stream) ||
+        hipMemcpyAsync(l.h_misc, misc_words(B), 12, hipMemcpyDeviceToHost, l.stream) ||
+        hipMemcpyAsync(l.h_misc + 4, B.run_off + B.n_tiles, 4, hipMemcpyDeviceToHost, l.stream))
         return RF_E_DEVICE;
     for (size_t i = 0; i < l.reqs.size(); ++i)
-        if (hipMemcpyAsync(l.pix[i], l.fbs[i]->color.p, l.pix_bytes[i], hipMemcpyDeviceToHost, l.stream))
+        if (hipMemcpyAsync(l.pix[i], fds[i].color, l.pix_bytes[i], hipMemcpyDeviceToHost, l.stream))
             return RF_E_DEVICE;
     l.h_misc[8] = 0;
     if (hipEventRecord(l.ev, l.stream)) return RF_E_DEVICE;
     return RF_OK;
 }
 
-// The worker never holds `m` across a HIP call: a stream's host function may
-// run on the calling thread when the stream is already idle, and it takes `m`.
+// The worker never holds `m` across a HIP call that may block on the device.
 static void worker_main(Serve* sp) {
     Serve& s = *sp;
     (void)hipSetDevice(s.dev->ordinal);
     std::unique_lock<std::mutex> lk(s.m);
     for (;;) {
         bool progressed = false;
-        std::vector<Lane*> relaunch, launch;
-        // 1. retire finished lanes
+        std::vector<Lane*> launch;
+        // 1. retire finished lanes; grow and rerun a launch that ran out of
+        //    clip-fan or bin-run capacity
         for (Lane* l : s.lanes) {
-            if (!l->busy || !l->finished.load()) continue;
+            if (!l->busy || l->wait_mem || !l->finished.load()) continue;
             progressed = true;
+            l->finished.store(0);
             if (l->h_misc[8]) {
                 finish_lane(s, *l, RF_E_DEVICE);
                 continue;
             }
             const uint32_t err = l->h_misc[2];
             if ((err & (ERRB_EXTRA | ERRB_RUNS)) && l->attempts < 3) {
-                // grow the lane's capacities to what the batch needed and rerun it
-                if (err & ERRB_EXTRA) l->extra_cap = l->h_misc[0] + l->h_misc[0] / 4 + 1024;
-                if (err & ERRB_RUNS) l->runs_cap = l->h_misc[4] + l->h_misc[4] / 4 + 1024;
+                const Plan p = plan_of(l->reqs);
+                if (err & ERRB_EXTRA) {
+                    const uint32_t need = l->h_misc[0];
+                    s.extra_ratio = std::max(s.extra_ratio, 1.25 * need / std::max<double>(1.0, (double)p.n_tris));
+                    l->extra_cap = std::max(l->extra_cap, grow_extra_cap(need));
+                }
+                if (err & ERRB_RUNS) {
+                    const uint32_t need = l->h_misc[4];
+                    const double groups = std::ceil((double)p.n_tris / WAVE_SIZE);
+                    s.runs_ratio = std::max(s.runs_ratio, 1.25 * need / std::max(1.0, groups));
+                    l->runs_cap = std::max(l->runs_cap, grow_runs_cap(need));
+                }
                 ++l->attempts;
-                l->finished.store(0);
-                relaunch.push_back(l);
+                s.arena.release(l->blk_off, l->blk_bytes);
+                l->blk_bytes = 0;
+                if (block_bytes(p, l->extra_cap, l->runs_cap) > s.arena.size) {
+                    l->done_s = now_s();
+                    finish_lane(s, *l, RF_E_BUDGET);
+                    continue;
+                }
+                l->wait_mem = true;  // takes its larger block below, or waits for one
                 continue;
             }
             finish_lane(s, *l, map_err(err));
         }
-        // 2. admit pending requests onto free lanes
+        // 2. reruns waiting for a larger block
         for (Lane* l : s.lanes) {
-            if (l->busy || s.pending.empty()) continue;
+            if (!l->wait_mem) continue;
+            if (take_block(s, *l)) {
+                l->wait_mem = false;
+                progressed = true;
+                launch.push_back(l);
+            } else {
+                bool others = false;
+                for (Lane* o : s.lanes) others = others || (o != l && o->busy && !o->wait_mem);
+                if (!others) {  // the arena is as free as it gets and still too fragmented or small
+                    l->done_s = now_s();
+                    finish_lane(s, *l, RF_E_BUDGET);
+                    progressed = true;
+                }
+            }
+        }
+        // 3. admit pending requests onto free lanes, in arrival order, while
+        //    the launch's block fits the largest free range of the arena (not
+        //    while a rerun waits for memory: it goes first)
+        bool rerun_waits = false;
+        for (Lane* l : s.lanes) rerun_waits = rerun_waits || l->wait_mem;
+        for (Lane* l : s.lanes) {
+            if (l->busy || s.pending.empty() || rerun_waits) continue;
+            const uint64_t room = s.arena.largest();
+            Plan p;
             while (!s.pending.empty() && l->reqs.size() < s.max_batch) {
                 Req& q = s.pending.front();
-                if (q.bytes > s.budget) {
-                    // can never fit: fail it
-                    Done d;
-                    std::memset(&d.res, 0, sizeof(d.res));
-                    d.res.id = q.r.id;
-                    d.res.status = RF_E_BUDGET;
-                    d.res.submit_s = q.submit_s;
-                    d.res.start_s = d.res.done_s = now_s();
-                    s.done.push_back(d);
+                Plan p2 = p;
+                p2.add(q);
+                Plan alone;
+                alone.add(q);
+                if (block_bytes(alone, est_extra(s, alone), est_runs(s, alone)) > s.arena.size) {
+                    reject(s, q, RF_E_BUDGET);  // can never fit
                     s.pending.pop_front();
-                    s.cv_done.notify_all();
+                    progressed = true;
                     continue;
                 }
-                if (s.used + q.bytes > s.budget) break;  // wait for memory to free up
-                s.used += q.bytes;
+                if (p2.n_tris >= MAX_LAUNCH_PRIMS) break;
+                if (block_bytes(p2, est_extra(s, p2), est_runs(s, p2)) > room) break;  // wait for memory
+                p = p2;
                 l->reqs.push_back(std::move(q));
                 s.pending.pop_front();
             }
             if (l->reqs.empty()) continue;
+            l->extra_cap = est_extra(s, p);
+            l->runs_cap = est_runs(s, p);
+            if (!take_block(s, *l)) {  // cannot happen: the block fits `room`
+                for (auto it = l->reqs.rbegin(); it != l->reqs.rend(); ++it) s.pending.push_front(std::move(*it));
+                l->reqs.clear();
+                continue;
+            }
             l->busy = true;
             l->attempts = 0;
             l->finished.store(0);
             progressed = true;
             launch.push_back(l);
         }
-        // 3. allocate targets and enqueue, without the lock
-        if (!launch.empty() || !relaunch.empty()) {
+        // 4. readback buffers and enqueue, without the lock
+        if (!launch.empty()) {
             lk.unlock();
             std::vector<std::pair<Lane*, rf_status>> failed;
             for (Lane* l : launch) {
                 rf_status st = RF_OK;
-                for (Req& q : l->reqs) {
-                    const size_t pb = (size_t)q.r.width * q.r.height * bytes_per_pixel(q.r.fmt);
-                    Framebuf* fb = fb_get(s, q.r.width, q.r.height, q.r.fmt, q.r.with_depth);
-                    uint8_t* pin = pin_get(s, pb);
-                    l->fbs.push_back(fb);
-                    l->pix.push_back(pin);
-                    l->pix_bytes.push_back(pb);
-                    if (!fb || !pin) st = RF_E_DEVICE;
-                }
+                if (l->pix.empty())
+                    for (Req& q : l->reqs) {
+                        const size_t pb = (size_t)q.r.width * q.r.height * bytes_per_pixel(q.r.fmt);
+                        uint8_t* pin = pin_get(s, pb);
+                        l->pix.push_back(pin);
+                        l->pix_bytes.push_back(pb);
+                        if (!pin) st = RF_E_DEVICE;
+                    }
                 if (st == RF_OK) st = launch_lane(s, *l);
-                if (st != RF_OK) failed.emplace_back(l, st);
-            }
-            for (Lane* l : relaunch) {
-                const rf_status st = launch_lane(s, *l);
-                if (st != RF_OK) failed.emplace_back(l, st);
+                if (st != RF_OK) {
+                    // work may be queued already: drain it before the block
+                    // and the readback buffers are reused
+                    (void)hipStreamSynchronize(l->stream);
+                    failed.emplace_back(l, st);
+                }
             }
             lk.lock();
             for (auto& f : failed) {
                 f.first->done_s = now_s();
                 finish_lane(s, *f.first, f.second);
             }
-            auto is_failed = [&](Lane* l) {
-                for (auto& f : failed)
-                    if (f.first == l) return true;
-                return false;
-            };
-            for (Lane* l : launch)
-                if (!is_failed(l)) {
+            for (Lane* l : launch) {
+                bool bad = false;
+                for (auto& f : failed) bad = bad || f.first == l;
+                if (!bad) {
                     l->launched = true;
                     l->cv_launch.notify_all();
                 }
-            for (Lane* l : relaunch)
-                if (!is_failed(l)) {
-                    l->launched = true;
-                    l->cv_launch.notify_all();
-                }
+            }
         }
         if (s.stop) {
             bool any = !s.pending.empty();
@@ -471,20 +616,22 @@ rf_status rf_serve_open(rf_device* dev, uint64_t budget_bytes, uint32_t max_batc
     Serve* s = new (std::nothrow) Serve();
     if (!s) return RF_E_DEVICE;
     s->dev = &d;
-    s->budget = budget_bytes;
     s->max_batch = max_batch;
+    budget_bytes &= ~(ALIGN - 1);
+    if (!budget_bytes || hipMalloc(&s->arena_mem, budget_bytes)) {
+        delete s;
+        return RF_E_DEVICE;
+    }
+    s->arena.init((uint8_t*)s->arena_mem, budget_bytes);
     for (uint32_t i = 0; i < n_streams; ++i) {
         Lane* l = new Lane();
+        s->lanes.push_back(l);
         if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) ||
             hipEventCreateWithFlags(&l->ev, hipEventBlockingSync | hipEventDisableTiming) ||
             hipHostMalloc((void**)&l->h_misc, 64, 0)) {
-            if (l->ev) (void)hipEventDestroy(l->ev);
-            if (l->stream) (void)hipStreamDestroy(l->stream);
-            delete l;
             rf_serve_close(reinterpret_cast<rf_serve*>(s));
             return RF_E_DEVICE;
         }
-        s->lanes.push_back(l);
     }
     for (Lane* l : s->lanes) l->waiter = std::thread(waiter_main, s, l);
     s->worker = std::thread(worker_main, s);
@@ -515,8 +662,8 @@ rf_status rf_serve_submit(rf_serve* sh, const rf_request* req) {
         q.n_tris += md.n_tris ? md.n_tris : m->n_tris - md.first_tri;
         q.n_verts += m->n_verts;
     }
+    if (q.n_tris >= MAX_LAUNCH_PRIMS) return RF_E_UNSUPPORTED;
     q.submit_s = now_s();
-    q.bytes = request_bytes(q);
     {
         std::lock_guard<std::mutex> g(s.m);
         if (s.stop) return RF_E_ARG;
@@ -569,8 +716,6 @@ rf_status rf_serve_close(rf_serve* sh) {
     (void)hipSetDevice(d.ordinal);
     for (Lane* l : s->lanes) {
         if (l->stream) (void)hipStreamSynchronize(l->stream);
-        l->sc.release(&s->bytes);
-        l->desc.release(&s->bytes);
         if (l->h_desc) (void)hipHostFree(l->h_desc);
         if (l->h_stats) (void)hipHostFree(l->h_stats);
         if (l->h_misc) (void)hipHostFree(l->h_misc);
@@ -578,12 +723,7 @@ rf_status rf_serve_close(rf_serve* sh) {
         if (l->stream) (void)hipStreamDestroy(l->stream);
         delete l;
     }
-    for (auto& kv : s->fb_pool)
-        for (Framebuf* f : kv.second) {
-            f->color.release(nullptr);
-            f->depth.release(nullptr);
-            delete f;
-        }
+    if (s->arena_mem) (void)hipFree(s->arena_mem);
     for (auto& d2 : s->done)
         if (d2.pixels) (void)hipHostFree(d2.pixels);
     for (auto& kv : s->pin_pool)

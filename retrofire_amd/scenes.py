@@ -323,3 +323,78 @@ def icospheres1080(n: int = 10000) -> Scene:
 
 
 SCENES = {"sphere640": sphere640, "crates1080": crates1080, "icospheres1080": icospheres1080}
+
+
+# ---------------------------------------------------------------------------
+# Batched and served workloads: frames and requests over the scenes above.
+# ---------------------------------------------------------------------------
+RESOLUTIONS = [(256, 256), (512, 512), (1280, 720), (1920, 1080), (2048, 2048)]
+
+
+def crates_mvp(w: int, h: int, yaw: float) -> np.ndarray:
+    """First person at (0, 1.7, 0) turned by `yaw` (turns), diagonal fov 90
+    degrees, near 0.1, far 1000, aspect w / h."""
+    aspect = F(w) / F(h)
+    proj = M.perspective(M.fov_diagonal_focal_ratio(90.0, aspect), aspect, 0.1, 1000.0)
+    return M.then(first_person(0.0, 1.7, 0.0, yaw), proj)
+
+
+def spheres_mvp(w: int, h: int, orbit: float) -> np.ndarray:
+    """Orbit of `orbit` turns about +y at distance 2.5 from the cloud's centre,
+    horizontal fov 60 degrees, near 0.1, far 100, aspect w / h."""
+    view = M.then(M.rotate_y(orbit), M.translate3(0.0, 0.0, 2.5))
+    return M.then(view, M.perspective(F(1.7320508), F(w) / F(h), 0.1, 100.0))
+
+
+def batch_frames(n: int, crate_tris: int, sphere_tris: int, tris_per_sphere: int, seed: int = 0):
+    """The batched workload's frames: n frames, each a resolution drawn
+    uniformly from RESOLUTIONS and a seeded mix of the crates (10-100 % of the
+    crates, 12 triangles each) and the icospheres (10-100 % of the spheres)
+    with random camera poses. Returns dicts of w, h, vp, mvp_c, mvp_i and the
+    triangle counts nc, ni of the two draws (prefixes of the merged meshes)."""
+    rng = Xorshift64(DEFAULT_SEED + 3 + seed)
+    out = []
+    for _ in range(n):
+        w, h = RESOLUTIONS[rng.uniform_usize(0, 5)]
+        fc = float(rng.uniform_f32(0.1, 1.0))
+        fi = float(rng.uniform_f32(0.1, 1.0))
+        yaw = float(rng.uniform_f32(0.0, 1.0))
+        orbit = float(rng.uniform_f32(0.0, 1.0))
+        nc = max(1, int(fc * crate_tris / 12)) * 12
+        ni = max(1, int(fi * sphere_tris / tris_per_sphere)) * tris_per_sphere
+        out.append(dict(w=w, h=h, vp=M.viewport(0, h, w, 0), mvp_c=crates_mvp(w, h, yaw),
+                        mvp_i=spheres_mvp(w, h, orbit), nc=nc, ni=ni))
+    return out
+
+
+def serve_meshes():
+    """The served workload's 16 resident meshes: 8 crate fields of 125, 250,
+    ..., 1000 crates (textured) and 8 icosphere clouds of 250, 500, ..., 2000
+    spheres (one colour each). Returns [(kind, pos, attr, tris)] and the crate
+    texture; kind is "crates" or "spheres"."""
+    out = []
+    tex = None
+    for k in range(8):
+        pos, uv, tris, t = crates_world(125 * (k + 1), seed=DEFAULT_SEED + 100 + k)
+        tex = t if tex is None else tex
+        out.append(("crates", pos, uv, tris))
+    for k in range(8):
+        pos, attr, tris, _, _ = icospheres_world(250 * (k + 1), seed=DEFAULT_SEED + 200 + k)
+        out.append(("spheres", pos, attr, tris))
+    return out, tex
+
+
+def serve_stream(rate: float, n: int, seed: int = 5):
+    """An open-loop request stream at `rate` requests/s: n requests of
+    (arrival time s, width, height, mesh index 0..15, pose in turns), with
+    exponential gaps, resolutions from RESOLUTIONS and meshes uniform over
+    serve_meshes()."""
+    rng = Xorshift64(DEFAULT_SEED + seed)
+    t, out = 0.0, []
+    for _ in range(n):
+        t += -math.log(max(1e-7, 1.0 - float(rng.uniform_f32(0.0, 1.0)))) / rate
+        w, h = RESOLUTIONS[rng.uniform_usize(0, 5)]
+        mi = rng.uniform_usize(0, 16)
+        pose = float(rng.uniform_f32(0.0, 1.0))
+        out.append((t, w, h, mi, pose))
+    return out

@@ -67,3 +67,29 @@ def test_no_device_without_gpu():
     if st == 0:
         pytest.skip("a GPU is visible")
     assert st == -4
+
+
+def test_launch_primitive_limit_is_refused_before_any_device_work():
+    """A draw of 2^29 primitives or more cannot be indexed by the rasterizer's
+    32-bit (index << 3 | fan) entries: rf_render refuses it with
+    RF_E_UNSUPPORTED while checking its arguments, before it touches the
+    device (the handle below is a zeroed stand-in that is never read)."""
+    import ctypes as C
+    import numpy as np
+    import retrofire_amd as rf
+    from retrofire_amd import _ffi
+    from retrofire_amd import mat as M
+    L = _ffi.lib()
+    fake_dev = C.create_string_buffer(4096)
+    tris = np.zeros(3, np.uint32)
+    pos = np.zeros(3, np.float32)
+    col = np.zeros((4, 4, 4), np.uint8)
+    keep = []
+    d = rf.make_draw(tris.reshape(1, 3), pos.reshape(1, 3), rf.shader.Flat(), M.identity(),
+                     M.viewport(0, 4, 4, 0), keep)
+    t = rf.make_target(rf.Colorbuf(col))
+    c = rf.Context().to_c()
+    for n, want in ((1 << 29, -5), ((1 << 29) + 12345, -5), (1 << 40, -5)):
+        d.n_tris = n
+        assert L.rf_render(C.cast(fake_dev, C.c_void_p), C.byref(d), C.byref(t), C.byref(c), None) == want
+    assert not col.any()

@@ -324,3 +324,62 @@ def test_largest_target_sides(w, h):
     o, g = run_both(sc.kind, sc.tris, sc.pos, sc.attr, sc.mvp, vp, w, h, ctx=rf.Context(clear=True))
     assert_same(o, g)
     assert g[2][6] > 1000
+
+
+@pytest.mark.parametrize("above", [40, 700])
+@pytest.mark.parametrize("kind", [1, 3])
+def test_rows_above_the_target_fold_onto_row_0(above, kind):
+    """A viewport reaching `above` rows over the target: every scanline above
+    it is drawn onto row 0 (`self.y as usize` saturates, raster.rs:106), in
+    row order, each with its own span, depth test and stats. 700 rows is more
+    than one rasterizer batch of edge records, so a triangle's folded rows go
+    in several parts."""
+    rng = np.random.default_rng(60 + above + kind)
+    w, h = 257, 180
+    pos, attr, tris, mvp, _ = random_soup(rng, 250, w, h, kind)
+    tex = rng.integers(0, 256, size=(32, 64, 4), dtype=np.uint8) if kind == 3 else None
+    vp = M.viewport(0, h, w, -above)
+    for clear in (True, False):
+        o, g = run_both(kind, tris, pos, attr, mvp, vp, w, h, tex=tex, ctx=rf.Context(clear=clear, face_cull=None))
+        assert_same(o, g)
+    assert g[2][5] > 10000
+
+
+def test_rows_entirely_above_the_target():
+    """Triangles whose every row lies above the target still draw their
+    folded spans onto row 0."""
+    w, h = 120, 40
+    pos = np.array([[-0.9, 2.0, 0.5], [0.8, 2.5, 0.5], [0.1, 4.0, 0.5], [-0.5, 1.2, 0.2], [0.9, 1.4, 0.2],
+                    [0.0, 3.0, 0.2]], np.float32)
+    tris = np.array([[0, 1, 2], [3, 4, 5]], np.uint32)
+    cols = np.random.default_rng(1).uniform(0, 1, size=(6, 3)).astype(np.float32)
+    mvp = M.scale3(1.0, 0.2, 1.0)  # y in NDC 0.24 .. 0.8, viewport maps NDC y 0 .. 1 to rows -200 .. -400
+    vp = M.viewport(0, 0, w, -400)
+    o, g = run_both(1, tris, pos, cols, mvp, vp, w, h, ctx=rf.Context(clear=True, face_cull=None))
+    assert_same(o, g)
+    assert g[2][6] > 0 and (g[0][1:] == (0, 0, 0, 255)).all()  # only row 0 is drawn
+
+
+def test_w_zero_vertex():
+    """A clip-space vertex (0, 0, 0, 0) lies on every frustum plane, so it is
+    inside and unclipped (clip.rs:101-111); its screen position is 0/0 = NaN.
+    The reference sorts the NaN y first (x86-64's NaN from 0/0 has its sign
+    set, total_cmp), rows whose left edge is NaN become spans 0..x1 with no
+    fragment (`as usize` of NaN is 0, `as u32` count 0) that still count in
+    frags.i (raster.rs:104-107, target.rs:176)."""
+    w, h = 160, 120
+    # w' = z: the vertex at the origin maps to (0, 0, 0, 0)
+    mvp = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0], [0, 0, 1, 0]], np.float32)
+    rng = np.random.default_rng(5)
+    pos = [[0, 0, 0], [0.5, 0.2, 1.0], [-0.3, 0.6, 1.0], [0.7, -0.4, 1.0], [0, 0, 0], [-0.6, -0.5, 1.0]]
+    pos += rng.uniform([-0.8, -0.8, 0.9], [0.8, 0.8, 1.0], size=(30, 3)).tolist()
+    pos = np.array(pos, np.float32)
+    tris = np.array([[0, 1, 2], [1, 0, 3], [3, 4, 5], [5, 2, 0]] + [[6 + 3 * i, 7 + 3 * i, 8 + 3 * i] for i in range(10)],
+                    np.uint32)
+    cols = rng.uniform(0, 1, size=(pos.shape[0], 3)).astype(np.float32)
+    vp = M.viewport(0, h, w, 0)
+    for kind, attr in ((1, cols), (2, cols[:, :2].copy())):
+        tex = rng.integers(0, 256, size=(8, 8, 4), dtype=np.uint8) if kind == 2 else None
+        o, g = run_both(kind, tris, pos, attr, mvp, vp, w, h, tex=tex, ctx=rf.Context(clear=True, face_cull=None))
+        assert_same(o, g)
+        assert g[2][5] > g[2][6]  # the NaN rows count in frags.i without writing
