@@ -104,7 +104,6 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import retrofire_amd as rf
-    from retrofire_amd import _ffi
     from retrofire_amd import batch as RB
     from retrofire_amd import scenes as S
 
@@ -161,7 +160,7 @@ def main():
         "data": "synthetic (seeded crates scene, resident in HBM)",
         "config": {"workload": "crates1080: 1000 textured crates (24k verts, 12k tris, 256x256 RGBA8, "
                                "SamplerRepeatPot), 1920x1080 RGBA8 + f32 depth, clear per frame",
-                   "frames_per_step": a.frames, "tile_w": batch.tile_w, "tile_h": int(_ffi.lib().rf_tile_height()),
+                   "frames_per_step": a.frames, "tile_w": batch.tile_w, "tile_h": batch.tile_h,
                    "chunk": batch.chunk,
                    "parallelism": "single GPU"},
         "mfrag_per_s": round(mfrag, 1),
@@ -175,6 +174,7 @@ def main():
                                        "and workload (profiles/k_raster_traffic.json), not measured in this run",
                      "alg_bytes_per_launch": alg_bytes, "launch_ms": round(raster_ms, 4)},
         "pipeline_alg_GBs": round(alg_frame_bytes * a.frames / (total_prof_ms * 1e-3) / 1e9, 1),
+        "autotune_ms": [list(t) for t in batch.tune_results()],
     }
 
     # plain drop-in call: host buffers, upload + run + readback per frame
@@ -359,7 +359,8 @@ def c3_run(rf, RB, a, W_steps):
            "triangles_per_step": int(sum(f["nc"] + f["ni"] for f in frames)),
            "pixels_per_step": int(sum(f["w"] * f["h"] for f in frames)),
            "frags_o_per_step": int(sum(s.frags_o for s in stats)),
-           "tile_w": b.tile_w, "chunk": b.chunk, "plan_s": round(plan_s, 2),
+           "tile_w": b.tile_w, "tile_h": b.tile_h, "chunk": b.chunk, "plan_s": round(plan_s, 2),
+           "autotune_ms": [list(t) for t in b.tune_results()],
            "stage_ms": {n: round(x, 4) for n, x in zip(["vertex", "setup", "scan", "bin", "binsort", "raster"], prof)},
            "raster_roofline": {"alg_bytes_per_launch": alg, "launch_ms": round(prof[5], 4),
                                "achieved_GBs": round(alg / (prof[5] * 1e-3) / 1e9, 1),

@@ -174,9 +174,15 @@ rf_status rf_device_set_tile_width(rf_device* dev, uint32_t tile_w);
  * width. No counterpart in the reference (a tuning knob of this
  * implementation); results are identical for both. */
 rf_status rf_device_set_raster_chunk(rf_device* dev, uint32_t chunk);
-/* Rasterizer tile height (rows) this library was built with. No counterpart
- * in the reference (an implementation parameter, reported by benchmarks). */
+/* Default rasterizer tile height (rows) of rf_render and of served launches.
+ * No counterpart in the reference (an implementation parameter, reported by
+ * benchmarks). */
 uint32_t rf_tile_height(void);
+/* Rasterizer tile height (16, 32 or 64 rows; 0 = the default, 32) used by
+ * rf_render and by batches created with a forced tile width on this device.
+ * No counterpart in the reference (a tuning knob of this implementation);
+ * results are identical for every height. */
+rf_status rf_device_set_tile_height(rf_device* dev, uint32_t tile_h);
 
 /* ---- plain path: the drop-in for render() ------------------------------
  * Synchronous, like the reference call: uploads the draw's data and the
@@ -243,10 +249,12 @@ typedef struct rf_frame {
 } rf_frame;
 
 typedef struct rf_batch rf_batch;
-/* Plans a batch: sizes scratch from one eager run, picks the tile width
- * and span chunk (tile_w = 0: autotune over widths 16, 32, 64 and chunks 6,
- * 8; 16, 32 or 64 to force the width, with the device's chunk, see
- * rf_device_set_raster_chunk) and captures the launch sequence
+/* Plans a batch: sizes scratch from one eager run, picks the rasterizer's
+ * tile shape and span chunk (tile_w = 0: autotune over tile widths 16, 32,
+ * 64, tile heights 16, 32, 64 and chunks 6, 8, each timed on the batch; 16,
+ * 32 or 64 to force the width, with the device's tile height and chunk, see
+ * rf_device_set_tile_height and rf_device_set_raster_chunk) and captures the
+ * launch sequence
  * into a HIP graph. The frames' descriptors are copied; the meshes, textures
  * and framebuffers must stay alive while the batch exists. */
 rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_frames,
@@ -258,6 +266,15 @@ rf_status rf_batch_wait(rf_batch* b, rf_stats* stats);
 rf_status rf_batch_free(rf_batch* b);
 uint32_t rf_batch_tile_width(const rf_batch* b);
 uint32_t rf_batch_raster_chunk(const rf_batch* b);
+uint32_t rf_batch_tile_height(const rf_batch* b);
+/* The autotuner's measurements: one entry per candidate it timed (none when
+ * the tile width was forced). Copies up to `max` entries and returns their
+ * number. */
+typedef struct rf_tune_entry {
+    uint32_t tile_w, tile_h, chunk;
+    float ms;   /* mean pipeline milliseconds of the batch (eager launches) */
+} rf_tune_entry;
+uint32_t rf_batch_tune_results(const rf_batch* b, rf_tune_entry* out, uint32_t max);
 /* Kernel-level timing of the last replay (HIP events on the batch stream):
  * total milliseconds and the rasterizer's share. */
 rf_status rf_batch_last_ms(const rf_batch* b, float* total_ms, float* raster_ms);

@@ -179,6 +179,11 @@ class Device:
         """Rasterizer tile width of render() on this device (16, 32, 64; 0 = default)."""
         check(_ffi.lib().rf_device_set_tile_width(self.h, tile_w))
 
+    def set_tile_height(self, tile_h: int = 0):
+        """Rasterizer tile height of render() and of batches with a forced tile
+        width on this device (16, 32, 64 rows; 0 = default, 32)."""
+        check(_ffi.lib().rf_device_set_tile_height(self.h, tile_h))
+
     def set_raster_chunk(self, chunk: int = 0):
         """Rasterizer span chunk of render() and of batches with a forced tile
         width on this device (6 or 8 pixels; 0 = default, 8)."""

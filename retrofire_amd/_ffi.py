@@ -63,6 +63,10 @@ class RfResult(C.Structure):
                 ("done_s", C.c_double), ("stats", RfStats)]
 
 
+class RfTuneEntry(C.Structure):
+    _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("chunk", C.c_uint32), ("ms", C.c_float)]
+
+
 # Every symbol declared in include/retrofire_amd.h, with its signature.
 _vp = C.c_void_p
 _st = C.c_int32
@@ -74,6 +78,7 @@ SIGNATURES = {
     "rf_device_set_tile_width": (_st, [_vp, C.c_uint32]),
     "rf_device_set_raster_chunk": (_st, [_vp, C.c_uint32]),
     "rf_tile_height": (C.c_uint32, []),
+    "rf_device_set_tile_height": (_st, [_vp, C.c_uint32]),
     "rf_render": (_st, [_vp, C.POINTER(RfDraw), C.POINTER(RfTarget), C.POINTER(RfCtx), C.POINTER(RfStats)]),
     "rf_mesh_create": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.c_uint64,
                              C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(_vp)]),
@@ -94,6 +99,8 @@ SIGNATURES = {
     "rf_batch_free": (_st, [_vp]),
     "rf_batch_tile_width": (C.c_uint32, [_vp]),
     "rf_batch_raster_chunk": (C.c_uint32, [_vp]),
+    "rf_batch_tile_height": (C.c_uint32, [_vp]),
+    "rf_batch_tune_results": (C.c_uint32, [_vp, C.POINTER(RfTuneEntry), C.c_uint32]),
     "rf_batch_last_ms": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "rf_batch_profile": (_st, [_vp, C.c_int, C.POINTER(C.c_float)]),
     "rf_serve_open": (_st, [_vp, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),

@@ -140,6 +140,17 @@ class Batch:
         return int(_ffi.lib().rf_batch_tile_width(self.h))
 
     @property
+    def tile_h(self):
+        return int(_ffi.lib().rf_batch_tile_height(self.h))
+
+    def tune_results(self):
+        """The autotuner's timings: [(tile_w, tile_h, chunk, ms)] per candidate."""
+        n = _ffi.lib().rf_batch_tune_results(self.h, None, 0)
+        arr = (_ffi.RfTuneEntry * max(1, n))()
+        _ffi.lib().rf_batch_tune_results(self.h, arr, n)
+        return [(e.tile_w, e.tile_h, e.chunk, round(e.ms, 4)) for e in arr[:n]]
+
+    @property
     def chunk(self):
         """The rasterizer's span chunk (6 or 8) this batch runs with."""
         return int(_ffi.lib().rf_batch_raster_chunk(self.h))

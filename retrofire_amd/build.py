@@ -22,6 +22,7 @@ SOURCES = [
     ("rf_api", "rf_api.cpp", []),
     ("rf_batch", "rf_batch.cpp", []),
     ("rf_serve", "rf_serve.cpp", []),
+    ("rf_dsort", "rf_dsort.cpp", []),
 ]
 HEADERS = ["rf_types.hpp", "rf_math.hpp", "rf_internal.hpp"]
 ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")

@@ -89,9 +89,9 @@ rf_status map_err(uint32_t e) {
 }
 
 // Frame tile grid for a tile width.
-void set_grid(FrameDesc& f, uint32_t tile_w, uint32_t& tile_base) {
+void set_grid(FrameDesc& f, uint32_t tile_w, uint32_t tile_h, uint32_t& tile_base) {
     f.ntx = (f.width + tile_w - 1) / tile_w;
-    f.nty = (f.height + TILE_ROWS - 1) / TILE_ROWS;
+    f.nty = (f.height + tile_h - 1) / tile_h;
     f.tile_base = tile_base;
     tile_base += f.ntx * f.nty;
 }
@@ -165,21 +165,24 @@ uint32_t grow_runs_cap(uint32_t needed) {
     return (uint32_t)std::min<uint64_t>(c, 0xFFFFFF00ull);
 }
 
-uint32_t raster_grid(const Device& d, uint32_t n_tiles) {
-    const uint32_t g = (uint32_t)d.num_cu * RASTER_WG_PER_CU;
-    return std::max(1u, std::min(n_tiles, g));
+uint32_t raster_grid(const Device& d, const BatchDesc& B) {
+    const uint32_t g = (uint32_t)d.num_cu * raster_wg_per_cu(B.tile_w, B.chunk, B.tile_h);
+    return std::max(1u, std::min(B.n_tiles, g));
 }
 
 // Runs the pipeline on `s` and waits; grows the clip-fan and run capacities
-// and runs again when they were exceeded. `restore` re-initialises anything
-// the aborted run may have written (the targets) before a rerun.
-rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const std::vector<FrameStats*>&,
-                    std::vector<FrameStats>& stats_out, uint32_t* err_out,
-                    const std::function<hipError_t()>& restore) {
+// and runs again when they were exceeded. `pre` (may be empty) enqueues what
+// must precede every run once B's scratch is carved (the depth-sorted fans);
+// `restore` re-initialises anything the aborted run may have written (the
+// targets) before a rerun.
+rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s,
+                    const std::function<hipError_t(const BatchDesc&)>& pre, std::vector<FrameStats>& stats_out,
+                    uint32_t* err_out, const std::function<hipError_t()>& restore) {
     if (B.n_tris >= MAX_LAUNCH_PRIMS) return RF_E_UNSUPPORTED;
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (sc.prepare(B, &d.bytes) != hipSuccess) return RF_E_DEVICE;
-        if (launch_pipeline(B, s, raster_grid(d, B.n_tiles)) != hipSuccess) return RF_E_DEVICE;
+        if (pre && pre(B) != hipSuccess) return RF_E_DEVICE;
+        if (launch_pipeline(B, s, raster_grid(d, B)) != hipSuccess) return RF_E_DEVICE;
         uint32_t misc[3];
         uint32_t total_runs = 0;
         if (hipMemcpyAsync(misc, misc_words(B), sizeof(misc), hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -294,7 +297,8 @@ hipError_t plain_download(const rf_target* target, const PlainLayout& L, const P
 }
 
 void plain_descs(const rf_draw* draw, const rf_target* target, const rf_ctx* ctx, const PlainLayout& L,
-                 const PlainBufs& P, uint32_t tile_w, uint32_t chunk, DrawDesc& dd, FrameDesc& fd, BatchDesc& B) {
+                 const PlainBufs& P, uint32_t tile_w, uint32_t tile_h, uint32_t chunk, DrawDesc& dd, FrameDesc& fd,
+                 BatchDesc& B) {
     std::memset(&dd, 0, sizeof(dd));
     dd.prim = draw->prim;
     dd.pos = (const float*)P.pos;
@@ -326,7 +330,7 @@ void plain_descs(const rf_draw* draw, const rf_target* target, const rf_ctx* ctx
     fd.fmt = target->fmt;
     fill_frame_ctx(fd, *ctx);
     uint32_t tile_base = 0;
-    set_grid(fd, tile_w, tile_base);
+    set_grid(fd, tile_w, tile_h, tile_base);
 
     std::memset(&B, 0, sizeof(B));
     B.draws = (const DrawDesc*)P.desc;
@@ -340,7 +344,7 @@ void plain_descs(const rf_draw* draw, const rf_target* target, const rf_ctx* ctx
     B.tile_w = tile_w;
     B.chunk = chunk;
     B.sort_tpw = 1;  // one frame: one tile per binsort wave costs little and spreads dense tiles
-    B.tile_h = TILE_ROWS;
+    B.tile_h = tile_h;
 }
 
 void plain_stats(rf_stats* stats, const rf_draw* draw, const FrameStats& fs, double t0) {
@@ -407,8 +411,9 @@ rf_status rf_device_close(rf_device* dev) {
     (void)hipSetDevice(d->ordinal);
     (void)hipStreamSynchronize(d->stream);
     d->plain.release(&d->bytes);
+    sortpass_release(d->ds, &d->bytes);
     for (DevBuf* b : {&d->up_pos, &d->up_attr, &d->up_tris, &d->up_tex, &d->up_color, &d->up_depth,
-                      &d->up_desc, &d->ds_fans, &d->ds_keys, &d->ds_vals, &d->ds_tmp, &d->ds_idx})
+                      &d->up_desc})
         b->release(&d->bytes);
     (void)hipStreamDestroy(d->stream);
     delete d;
@@ -419,7 +424,13 @@ uint64_t rf_device_bytes(const rf_device* dev) {
     return dev ? reinterpret_cast<const Device*>(dev)->bytes : 0;
 }
 
-uint32_t rf_tile_height(void) { return TILE_ROWS; }
+uint32_t rf_tile_height(void) { return DEFAULT_TILE_H; }
+
+rf_status rf_device_set_tile_height(rf_device* dev, uint32_t tile_h) {
+    if (!dev || (tile_h != 0 && !valid_tile_h(tile_h))) return RF_E_ARG;
+    reinterpret_cast<Device*>(dev)->plain_tile_h = tile_h ? tile_h : DEFAULT_TILE_H;
+    return RF_OK;
+}
 
 rf_status rf_device_set_tile_width(rf_device* dev, uint32_t tile_w) {
     if (!dev || (tile_w != 0 && tile_w != 16 && tile_w != 32 && tile_w != 64)) return RF_E_ARG;
@@ -460,7 +471,7 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     DrawDesc dd;
     FrameDesc fd;
     BatchDesc B;
-    plain_descs(draw, target, ctx, L, P, d.plain_tile_w, d.plain_chunk, dd, fd, B);
+    plain_descs(draw, target, ctx, L, P, d.plain_tile_w, d.plain_tile_h, d.plain_chunk, dd, fd, B);
     uint8_t* desc = (uint8_t*)P.desc;
     if (hipMemcpyAsync(desc, &dd, sizeof(dd), hipMemcpyHostToDevice, s) ||
         hipMemcpyAsync((void*)B.frames, &fd, sizeof(fd), hipMemcpyHostToDevice, s))
@@ -472,66 +483,42 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
     // device and drawn in that order as unclipped triangles. Edges keep their
     // order: Render::depth is +inf for every edge (render.rs:67-70), and
     // sort_unstable_by leaves a slice of equal keys as it is.
+    std::function<hipError_t(const BatchDesc&)> pre;
     if (ctx->depth_sort && draw->prim == PRIM_TRI && draw->n_tris) {
-        uint32_t cap = std::max<uint32_t>(d.ds_cap, (uint32_t)std::min<uint64_t>(2 * draw->n_tris + 1024, 1u << 30));
-        uint32_t n_f = 0;
-        for (int attempt = 0;; ++attempt) {
-            if (attempt == 4) return RF_E_INTERNAL;
-            if (d.plain.prepare(B, &d.bytes) || d.ds_fans.ensure((size_t)cap * FAN_FLOATS * 4, &d.bytes) ||
-                d.ds_keys.ensure((size_t)cap * 16, &d.bytes) || d.ds_vals.ensure((size_t)cap * 8, &d.bytes))
-                return RF_E_DEVICE;
-            BatchDesc A = B;
-            A.depth_sort = (uint32_t)ctx->depth_sort;
-            A.fan_cap = cap;
-            A.fans = (float*)d.ds_fans.p;
-            A.fan_keys = (unsigned long long*)d.ds_keys.p;
-            A.fan_vals = (uint32_t*)d.ds_vals.p;
-            A.fan_count = misc_words(A) + 32;
-            uint32_t misc[33];
-            if (launch_fans(A, s) ||
-                hipMemcpyAsync(misc, misc_words(A), sizeof(misc), hipMemcpyDeviceToHost, s) ||
-                hipStreamSynchronize(s))
-                return RF_E_DEVICE;
-            const uint32_t e = misc[2];
-            if (e & ERRB_INDEX) return RF_E_INDEX;
-            if (e & ERRB_CLIP) return RF_E_INTERNAL;
-            if (e & ERRB_EXTRA) {
-                if (cap >= (1u << 30)) return RF_E_UNSUPPORTED;
-                cap = (uint32_t)std::min<uint64_t>((uint64_t)misc[32] + misc[32] / 4 + 1024, 1u << 30);
-                continue;
-            }
-            n_f = misc[32];
-            break;
-        }
-        d.ds_cap = cap;
-        unsigned long long* keys = (unsigned long long*)d.ds_keys.p;
-        uint32_t* vals = (uint32_t*)d.ds_vals.p;
-        size_t tb = 0;
-        if (sort_pairs_u64(nullptr, tb, keys, keys + cap, vals, vals + cap, n_f, s) ||
-            d.ds_tmp.ensure(std::max<size_t>(tb, 4), &d.bytes) ||
-            sort_pairs_u64(d.ds_tmp.p, tb, keys, keys + cap, vals, vals + cap, n_f, s) ||
-            d.ds_idx.ensure(std::max<size_t>((size_t)n_f * 12, 4), &d.bytes))
-            return RF_E_DEVICE;
+        SortPass& S = d.ds;
+        S.draws.assign(1, dd);
+        S.draws[0].sort = (uint32_t)ctx->depth_sort;
+        S.main_of.assign(1, 0);
+        S.frames = B.frames;
+        S.frames_n = 1;
+        uint32_t e = 0;
+        if ((st = sortpass_plan(d, S, s, &d.bytes, &e))) return st;
+        if (e & ERRB_INDEX) return RF_E_INDEX;
+        if (e & ERRB_CLIP) return RF_E_INTERNAL;
+        if (sortpass_map(S, s)) return RF_E_DEVICE;
         // the sorted fans as vertex slots and index triples of one draw
+        const uint32_t n_f = S.n_f;
         dd.pre = 1;
+        dd.in_tris = dd.n_tris;
+        dd.in_verts = dd.n_verts;
         dd.n_verts = 3 * n_f;
         dd.n_tris = n_f;
-        dd.tris = (const uint32_t*)d.ds_idx.p;
+        dd.tris = sortpass_idx(S, 0);
         if (hipMemcpyAsync(desc, &dd, sizeof(dd), hipMemcpyHostToDevice, s)) return RF_E_DEVICE;
         B.n_verts = 3ull * n_f;
         B.n_tris = n_f;
         B.n_groups = (uint32_t)((n_f + WAVE_SIZE - 1) / WAVE_SIZE);
         B.runs_cap = std::max<uint32_t>(B.runs_cap, 4 * B.n_groups + B.n_tiles + 1024);
-        B.fans = (float*)d.ds_fans.p;  // the gather reads the fans of the first launch
-        if (d.plain.prepare(B, &d.bytes) ||
-            launch_fan_gather(B, vals + cap, n_f, (uint32_t*)d.ds_idx.p, s))
-            return RF_E_DEVICE;
-        B.fans = nullptr;
+        // P's fans again and their sort into the (re)carved scratch, before every run
+        pre = [&S, s](const BatchDesc& M) -> hipError_t {
+            hipError_t r = sortpass_fans(S, s);
+            return r ? r : sortpass_sort_gather(S, M, s);
+        };
     }
 
     std::vector<FrameStats> fs;
     uint32_t err = 0;
-    st = run_sized(d, d.plain, B, s, {}, fs, &err, upload_target);
+    st = run_sized(d, d.plain, B, s, pre, fs, &err, upload_target);
     if (st) return st;
     d.plain_extra_cap = std::max(d.plain_extra_cap, B.extra_cap);
     d.plain_runs_cap = std::max(d.plain_runs_cap, B.runs_cap);

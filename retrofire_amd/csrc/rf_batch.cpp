@@ -29,7 +29,7 @@ struct Batch {
     DevBuf d_desc;
     Scratch sc;
     BatchDesc B{};
-    uint32_t tile_w = 64;
+    uint32_t tile_w = 64, tile_h = DEFAULT_TILE_H;  // rasterizer tile shape
     uint32_t chunk = 8;  // rasterizer span chunk (6 or 8)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -37,9 +37,43 @@ struct Batch {
     FrameStats* h_stats = nullptr;  // pinned
     uint32_t* h_misc = nullptr;     // pinned
     bool pending = false;
+    std::vector<rf_tune_entry> tune;  // the autotuner's timings (tile width, height, chunk, ms)
+    // depth sort: the triangle draws of frames whose context asks for it
+    SortPass sp;
+    std::vector<int32_t> sort_of;     // per draw (in batch order): its draw in sp, or -1
 };
 
-static rf_status build_descs(Batch& b, uint32_t tile_w, uint32_t chunk) {
+// The device descriptor of mesh draw `md` of frame `fi` (draw order
+// positions vert_base / tri_base are the caller's).
+static DrawDesc draw_desc(const rf_mesh_draw& md, uint32_t fi) {
+    const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
+    const Tex* t = reinterpret_cast<const Tex*>(md.tex);
+    DrawDesc dd;
+    std::memset(&dd, 0, sizeof(dd));
+    const uint64_t first = md.first_tri;
+    const uint64_t nt = md.n_tris ? md.n_tris : m->n_tris - first;
+    dd.pos = (const float*)m->pos.p;
+    dd.attr = (const float*)m->attr.p;
+    dd.tris = (const uint32_t*)m->tris.p + (m->prim == PRIM_EDGE ? 2 : 3) * first;
+    dd.prim = m->prim;
+    dd.tex = t ? t->texels() : nullptr;
+    dd.n_verts = (uint32_t)m->n_verts;
+    dd.n_tris = (uint32_t)nt;
+    dd.kind = md.kind;
+    dd.frame = fi;
+    dd.tex_w = t ? t->w : 0;
+    dd.tex_h = t ? t->h : 0;
+    dd.tex_stride = t ? t->stride : 0;
+    dd.flat_rgba = pack_pixel(PIX_RGBA8, md.flat_color[0], md.flat_color[1], md.flat_color[2], md.flat_color[3]);
+    dd.attr_stride = m->n_attr;
+    std::memcpy(dd.mvp, md.mvp, 64);
+    std::memcpy(dd.vp, md.viewport, 64);
+    dd.cull = md.cull;
+    std::memcpy(dd.bbox, m->bbox, sizeof(dd.bbox));
+    return dd;
+}
+
+static rf_status build_descs(Batch& b, uint32_t tile_w, uint32_t tile_h, uint32_t chunk) {
     Device& d = *b.dev;
     b.h_draws.clear();
     b.h_frames.clear();
@@ -63,47 +97,40 @@ static rf_status build_descs(Batch& b, uint32_t tile_w, uint32_t chunk) {
         fd.height = fb->h;
         fd.fmt = fb->fmt;
         fill_frame_ctx(fd, f.ctx);
-        set_grid(fd, tile_w, tile_base);
+        set_grid(fd, tile_w, tile_h, tile_base);
         b.h_frames.push_back(fd);
         for (const rf_mesh_draw& md : b.draws[fi]) {
             const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
-            const Tex* t = reinterpret_cast<const Tex*>(md.tex);
-            DrawDesc dd;
-            std::memset(&dd, 0, sizeof(dd));
-            const uint64_t first = md.first_tri;
-            const uint64_t nt = md.n_tris ? md.n_tris : m->n_tris - first;
-            dd.pos = (const float*)m->pos.p;
-            dd.attr = (const float*)m->attr.p;
-            dd.tris = (const uint32_t*)m->tris.p + (m->prim == PRIM_EDGE ? 2 : 3) * first;
-            dd.prim = m->prim;
-            dd.tex = t ? t->texels() : nullptr;
-            dd.n_verts = (uint32_t)m->n_verts;
-            dd.n_tris = (uint32_t)nt;
-            dd.kind = md.kind;
-            dd.frame = (uint32_t)fi;
-            dd.tex_w = t ? t->w : 0;
-            dd.tex_h = t ? t->h : 0;
-            dd.tex_stride = t ? t->stride : 0;
-            dd.flat_rgba = pack_pixel(PIX_RGBA8, md.flat_color[0], md.flat_color[1], md.flat_color[2],
-                                      md.flat_color[3]);
-            dd.attr_stride = m->n_attr;
-            dd.vert_base = vbase;
-            dd.tri_base = tbase;
-            std::memcpy(dd.mvp, md.mvp, 64);
-            std::memcpy(dd.vp, md.viewport, 64);
-            dd.cull = md.cull;
-            std::memcpy(dd.bbox, m->bbox, sizeof(dd.bbox));
-            any_cull |= md.cull;
-            vbase += m->n_verts;
-            tbase += nt;
+            DrawDesc dd = draw_desc(md, (uint32_t)fi);
+            const uint64_t nt = dd.n_tris;
             b.frame_prims_i[fi] += nt;
             b.frame_verts_i[fi] += m->n_verts;
+            const int32_t si = b.sort_of[b.h_draws.size()];
+            if (si >= 0) {
+                // a depth-sorted render() call: its sorted clip fans, 3 vertex
+                // slots each, filled by the sort pass's gather
+                const uint32_t nf = b.sp.cnt[si];
+                dd.pre = 1;
+                dd.sort = (uint32_t)f.ctx.depth_sort;
+                dd.in_tris = dd.n_tris;
+                dd.in_verts = dd.n_verts;
+                dd.n_tris = nf;
+                dd.n_verts = 3 * nf;
+                dd.tris = sortpass_idx(b.sp, b.sp.off[si]);
+                b.sp.main_of[si] = (uint32_t)b.h_draws.size();
+            }
+            dd.vert_base = vbase;
+            dd.tri_base = tbase;
+            any_cull |= md.cull;
+            vbase += dd.n_verts;
+            tbase += dd.n_tris;
             b.frame_calls[fi] += 1;
             b.frame_objs[fi] += md.cull;
             b.h_draws.push_back(dd);
         }
     }
     if (tbase >= MAX_LAUNCH_PRIMS) return RF_E_UNSUPPORTED;
+    if (!b.sp.draws.empty() && sortpass_map(b.sp, b.stream)) return RF_E_DEVICE;
     const size_t db = b.h_draws.size() * sizeof(DrawDesc), fb = b.h_frames.size() * sizeof(FrameDesc);
     const size_t foff = (db + 255) & ~size_t(255);
     if (b.d_desc.ensure(foff + fb + 256, &d.bytes)) return RF_E_DEVICE;
@@ -121,21 +148,31 @@ static rf_status build_descs(Batch& b, uint32_t tile_w, uint32_t chunk) {
     B.n_tiles = tile_base;
     B.any_cull = any_cull;
     B.tile_w = tile_w;
-    B.tile_h = TILE_ROWS;
+    B.tile_h = tile_h;
     B.chunk = chunk;
     B.extra_cap = std::max<uint32_t>(b.B.extra_cap, 1024u);
     B.runs_cap = std::max<uint32_t>(b.B.runs_cap, 4 * B.n_groups + B.n_tiles + 1024);
     b.B = B;
     b.tile_w = tile_w;
+    b.tile_h = tile_h;
     b.chunk = chunk;
     return RF_OK;
+}
+
+// The depth sort pass (if the batch has one): P's fans, their sorts and the
+// gather into the batch's current scratch.
+static hipError_t sort_pass(Batch& b, const BatchDesc& M) {
+    if (b.sp.draws.empty()) return hipSuccess;
+    hipError_t e = sortpass_fans(b.sp, b.stream);
+    return e ? e : sortpass_sort_gather(b.sp, M, b.stream);
 }
 
 // Eager run that sizes the scratch exactly for this batch (with headroom).
 static rf_status size_batch(Batch& b) {
     std::vector<FrameStats> fs;
     uint32_t err = 0;
-    rf_status st = run_sized(*b.dev, b.sc, b.B, b.stream, {}, fs, &err, nullptr);
+    rf_status st = run_sized(*b.dev, b.sc, b.B, b.stream, [&b](const BatchDesc& M) { return sort_pass(b, M); },
+                             fs, &err, nullptr);
     if (st) return st;
     if ((st = map_err(err))) return st;
     return RF_OK;
@@ -144,7 +181,7 @@ static rf_status size_batch(Batch& b) {
 static float time_eager(Batch& b, int reps) {
     if (hipEventRecord(b.ev0, b.stream)) return -1.0f;
     for (int i = 0; i < reps; ++i)
-        if (launch_pipeline(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles))) return -1.0f;
+        if (sort_pass(b, b.B) || launch_pipeline(b.B, b.stream, raster_grid(*b.dev, b.B))) return -1.0f;
     if (hipEventRecord(b.ev1, b.stream) || hipEventSynchronize(b.ev1)) return -1.0f;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, b.ev0, b.ev1)) return -1.0f;
@@ -161,7 +198,8 @@ static void destroy_graph(Batch& b) {
 static rf_status capture(Batch& b) {
     destroy_graph(b);
     if (hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal)) return RF_E_DEVICE;
-    const hipError_t le = launch_pipeline(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles));
+    hipError_t le = sort_pass(b, b.B);
+    if (!le) le = launch_pipeline(b.B, b.stream, raster_grid(*b.dev, b.B));
     const hipError_t ee = hipStreamEndCapture(b.stream, &b.graph);
     if (le || ee || !b.graph) return RF_E_DEVICE;
     if (hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0)) return RF_E_DEVICE;
@@ -175,6 +213,7 @@ static void batch_release(Batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     destroy_graph(*b);
     b->sc.release(&d.bytes);
+    sortpass_release(b->sp, &d.bytes);
     b->d_desc.release(&d.bytes);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -199,7 +238,6 @@ rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_fra
         const rf_frame& f = frames[i];
         if (!f.target || (f.n_draws && !f.draws)) return RF_E_ARG;
         rf_status st = check_ctx(&f.ctx);
-        if (!st && f.ctx.depth_sort) st = RF_E_UNSUPPORTED;  // depth sort: rf_render only
         if (st) return st;
         for (uint32_t k = 0; k < f.n_draws; ++k) {
             const rf_mesh_draw& md = f.draws[k];
@@ -229,18 +267,50 @@ rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_fra
         batch_release(b);
         return RF_E_DEVICE;
     }
-    // Tile width and span chunk: the tile width forced (with the device's
-    // chunk), or the fastest (width, chunk) pair on this batch.
-    std::vector<std::pair<uint32_t, uint32_t>> cands;
+    std::memset(b->h_misc, 0, 64);
+    // Depth sort (render.rs:180-183): the triangle draws of frames that ask
+    // for it go through the sort pass; its planning run gives their fan counts.
+    for (uint32_t i = 0; i < n_frames; ++i)
+        for (const rf_mesh_draw& md : b->draws[i]) {
+            const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
+            if (frames[i].ctx.depth_sort && m->prim == PRIM_TRI) {
+                b->sort_of.push_back((int32_t)b->sp.draws.size());
+                DrawDesc dd = draw_desc(md, i);
+                dd.sort = (uint32_t)frames[i].ctx.depth_sort;
+                b->sp.draws.push_back(dd);
+            } else {
+                b->sort_of.push_back(-1);
+            }
+        }
+    if (!b->sp.draws.empty()) {
+        b->sp.frames_n = n_frames;
+        b->sp.main_of.assign(b->sp.draws.size(), 0);
+        uint32_t e = 0;
+        st = sortpass_plan(d, b->sp, b->stream, &d.bytes, &e);
+        if (!st) st = map_err(e);
+        if (st) {
+            batch_release(b);
+            return st;
+        }
+    }
+    // Tile shape and span chunk: the tile width forced (with the device's
+    // tile height and chunk), or the fastest (width, height, chunk) of the
+    // 3 x 3 x 2 candidates, each timed on this batch.
+    struct Cand {
+        uint32_t w, h, c;
+    };
+    std::vector<Cand> cands;
     if (tile_w)
-        cands.push_back({tile_w, d.plain_chunk});
+        cands.push_back({tile_w, d.plain_tile_h, d.plain_chunk});
     else
         for (uint32_t w : {16u, 32u, 64u})
-            for (uint32_t c : {6u, 8u}) cands.push_back({w, c});
+            for (uint32_t h : {16u, 32u, 64u})
+                for (uint32_t c : {6u, 8u}) cands.push_back({w, h, c});
     float best = 1e30f;
-    std::pair<uint32_t, uint32_t> best_wc = cands[0];
-    for (const auto& wc : cands) {
-        if ((st = build_descs(*b, wc.first, wc.second)) || (st = size_batch(*b))) {
+    Cand bc = cands[0];
+    b->tune.clear();
+    for (const Cand& c : cands) {
+        if ((st = build_descs(*b, c.w, c.h, c.c)) || (st = size_batch(*b))) {
             batch_release(b);
             return st;
         }
@@ -250,14 +320,15 @@ rf_status rf_batch_create(rf_device* dev, const rf_frame* frames, uint32_t n_fra
                 batch_release(b);
                 return RF_E_DEVICE;
             }
+            b->tune.push_back({c.w, c.h, c.c, ms});
             if (ms < best) {
                 best = ms;
-                best_wc = wc;
+                bc = c;
             }
         }
     }
-    if ((best_wc.first != b->tile_w || best_wc.second != b->chunk) &&
-        ((st = build_descs(*b, best_wc.first, best_wc.second)) || (st = size_batch(*b)))) {
+    if ((bc.w != b->tile_w || bc.h != b->tile_h || bc.c != b->chunk) &&
+        ((st = build_descs(*b, bc.w, bc.h, bc.c)) || (st = size_batch(*b)))) {
         batch_release(b);
         return st;
     }
@@ -295,6 +366,10 @@ rf_status rf_batch_run(rf_batch* bh, int sync) {
                        b.stream) ||
         hipMemcpyAsync(b.h_misc, misc_words(b.B), 16, hipMemcpyDeviceToHost, b.stream))
         return RF_E_DEVICE;
+    // the sort pass's error bits (its own counters)
+    if (!b.sp.draws.empty() &&
+        hipMemcpyAsync(b.h_misc + 8, misc_words(b.sp.P) + 2, 4, hipMemcpyDeviceToHost, b.stream))
+        return RF_E_DEVICE;
     b.pending = true;
     if (sync) return rf_batch_wait(bh, nullptr);
     return RF_OK;
@@ -305,7 +380,7 @@ rf_status rf_batch_wait(rf_batch* bh, rf_stats* stats) {
     Batch& b = *reinterpret_cast<Batch*>(bh);
     if (hipStreamSynchronize(b.stream)) return RF_E_DEVICE;
     b.pending = false;
-    rf_status st = map_err(b.h_misc[2]);
+    rf_status st = map_err(b.h_misc[2] | b.h_misc[8]);
     if (st) return st;
     if (stats) {
         float ms = 0.0f;
@@ -342,6 +417,18 @@ uint32_t rf_batch_raster_chunk(const rf_batch* bh) {
     return bh ? reinterpret_cast<const Batch*>(bh)->chunk : 0;
 }
 
+uint32_t rf_batch_tile_height(const rf_batch* bh) {
+    return bh ? reinterpret_cast<const Batch*>(bh)->tile_h : 0;
+}
+
+uint32_t rf_batch_tune_results(const rf_batch* bh, rf_tune_entry* out, uint32_t max) {
+    if (!bh) return 0;
+    const Batch& b = *reinterpret_cast<const Batch*>(bh);
+    const uint32_t n = (uint32_t)std::min<size_t>(b.tune.size(), max);
+    for (uint32_t i = 0; i < n && out; ++i) out[i] = b.tune[i];
+    return (uint32_t)b.tune.size();
+}
+
 rf_status rf_batch_last_ms(const rf_batch* bh, float* total_ms, float* raster_ms) {
     if (!bh) return RF_E_ARG;
     const Batch& b = *reinterpret_cast<const Batch*>(bh);
@@ -362,7 +449,7 @@ rf_status rf_batch_profile(rf_batch* bh, int iters, float* ms) {
     double acc[6] = {0, 0, 0, 0, 0, 0};
     rf_status st = RF_OK;
     for (int it = 0; it < iters && !st; ++it) {
-        if (launch_pipeline_timed(b.B, b.stream, raster_grid(*b.dev, b.B.n_tiles), ev) ||
+        if (launch_pipeline_timed(b.B, b.stream, raster_grid(*b.dev, b.B), ev) ||
             hipEventSynchronize(ev[6])) {
             st = RF_E_DEVICE;
             break;

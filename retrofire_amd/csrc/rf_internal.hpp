@@ -24,11 +24,8 @@ constexpr uint64_t MAX_LAUNCH_PRIMS = 1ull << 29;
 // Clip-fan and bin-run capacities grown by a rerun that needs more.
 uint32_t grow_extra_cap(uint32_t needed);
 uint32_t grow_runs_cap(uint32_t needed);
-constexpr uint32_t TILE_ROWS = TILE_HEIGHT;  // rasterizer tile height
-#ifndef RF_RASTER_WG_PER_CU
-#define RF_RASTER_WG_PER_CU (TILE_HEIGHT > 32 ? 4 : 8)
-#endif
-constexpr uint32_t RASTER_WG_PER_CU = RF_RASTER_WG_PER_CU;  // persistent rasterizer workgroups per CU
+constexpr uint32_t DEFAULT_TILE_H = 32;  // rasterizer tile height of rf_render and served launches
+inline bool valid_tile_h(uint32_t th) { return th == 16 || th == 32 || th == 64; }
 
 // Grow-only device allocation.
 struct DevBuf {
@@ -55,6 +52,31 @@ struct Scratch {
     void release(uint64_t* counter) { mem.release(counter); }
 };
 
+// Depth sort of some render() calls of a batch (render.rs:180-183,
+// 209-219; rf_dsort.cpp). Their draws, over their own meshes, form a batch P
+// of their own: P's pass clips every triangle into clip fans with depth keys,
+// a radix sort orders the fans by depth (then triangle and fan, so equal
+// depths keep submission order), a second stable sort groups them by draw,
+// and the gather writes each draw's fans, in that order, as the vertex slots
+// and index triples of the corresponding pre draw of the main batch. The
+// planning run learns each draw's fan count (fixed for fixed inputs), which
+// sizes the main batch's pre draws.
+struct SortPass {
+    std::vector<DrawDesc> draws;      // P's draws (host copies; vert_base / tri_base in P)
+    std::vector<uint32_t> main_of;    // P draw -> its draw in the main batch
+    std::vector<uint32_t> cnt, off;   // per P draw: fans, first position in the sorted order
+    uint32_t n_f = 0;                 // fans in all
+    uint32_t cap = 0;                 // fan capacity of P's pass
+    uint32_t sort_bits = 0;           // key bits of the second sort (0: one draw, none)
+    size_t tmp_bytes = 0;
+    BatchDesc P{};
+    Scratch sc;
+    DevBuf desc, fans, keys, vals, fdraw, fcnt, key2, tmp, idx, map;
+    uint32_t frames_n = 0;            // main frames (P's draws keep their frame indices)
+    const FrameDesc* frames = nullptr;
+};
+
+
 struct Device {
     int ordinal = 0;
     int num_cu = 256;
@@ -62,9 +84,9 @@ struct Device {
     uint64_t bytes = 0;  // device memory held
     Scratch plain;
     DevBuf up_pos, up_attr, up_tris, up_tex, up_color, up_depth, up_desc;
-    DevBuf ds_fans, ds_keys, ds_vals, ds_tmp, ds_idx;  // depth sort: clip fans, keys, order, sort scratch
-    uint32_t ds_cap = 0;
+    SortPass ds;  // rf_render's depth sort
     uint32_t plain_tile_w = 32;  // rasterizer tile width of rf_render
+    uint32_t plain_tile_h = DEFAULT_TILE_H;  // and its tile height
     uint32_t plain_chunk = 8;    // rasterizer span chunk of rf_render and of batches with a forced tile width
     uint32_t plain_extra_cap = 0, plain_runs_cap = 0;
 };
@@ -98,11 +120,15 @@ void fill_frame_ctx(FrameDesc& f, const rf_ctx& c);
 rf_status check_ctx(const rf_ctx* c);
 rf_status check_tex(uint32_t kind, const void* tex, uint32_t w, uint32_t h, uint32_t stride);
 rf_status map_err(uint32_t e);
-void set_grid(FrameDesc& f, uint32_t tile_w, uint32_t& tile_base);
-uint32_t raster_grid(const Device& d, uint32_t n_tiles);
-rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s, const std::vector<FrameStats*>&,
-                    std::vector<FrameStats>& stats_out, uint32_t* err_out,
-                    const std::function<hipError_t()>& restore);
+void set_grid(FrameDesc& f, uint32_t tile_w, uint32_t tile_h, uint32_t& tile_base);
+// Persistent rasterizer workgroups for batch B (resident workgroups per CU of
+// its instance, times the CUs, at most one per tile).
+uint32_t raster_grid(const Device& d, const BatchDesc& B);
+// Resident one-wave rasterizer workgroups per CU of an instance (rf_kernels.hip).
+uint32_t raster_wg_per_cu(uint32_t tile_w, uint32_t chunk, uint32_t tile_h);
+rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s,
+                    const std::function<hipError_t(const BatchDesc&)>& pre, std::vector<FrameStats>& stats_out,
+                    uint32_t* err_out, const std::function<hipError_t()>& restore);
 
 // The plain path (one render() call with host buffers): argument checks,
 // the device layout of its inputs and target, the host<->device copies, the
@@ -126,18 +152,40 @@ hipError_t plain_upload_target(const rf_target* target, const rf_ctx* ctx, const
                                hipStream_t s);
 hipError_t plain_download(const rf_target* target, const PlainLayout& L, const PlainBufs& P, hipStream_t s);
 void plain_descs(const rf_draw* draw, const rf_target* target, const rf_ctx* ctx, const PlainLayout& L,
-                 const PlainBufs& P, uint32_t tile_w, uint32_t chunk, DrawDesc& dd, FrameDesc& fd, BatchDesc& B);
+                 const PlainBufs& P, uint32_t tile_w, uint32_t tile_h, uint32_t chunk, DrawDesc& dd, FrameDesc& fd,
+                 BatchDesc& B);
 void plain_stats(rf_stats* stats, const rf_draw* draw, const FrameStats& fs, double t0);
 
-// Depth sort (rf_kernels.hip, rf_sort.hip): clip fans and their keys for the
-// single draw of `B`; the sort of (key, fan) pairs; the sorted fans written
-// as vertex slots 3i..3i+2 with index triples `idx`.
+// Depth sort kernels (rf_kernels.hip, rf_sort.hip): the clip fans of B's
+// draws and their keys (B.fan_*); the radix sorts of (key, fan) pairs; the
+// draw of each fan in an order; the sorted fans written as vertex slots of
+// the main batch M's depth-sorted draws, with their index triples.
 hipError_t launch_fans(const BatchDesc& B, hipStream_t s);
 hipError_t sort_pairs_u64(void* tmp, size_t& tmp_bytes, const unsigned long long* keys_in,
                           unsigned long long* keys_out, const uint32_t* vals_in, uint32_t* vals_out, uint32_t n,
                           hipStream_t s);
-hipError_t launch_fan_gather(const BatchDesc& B, const uint32_t* order, uint32_t n_fans, uint32_t* idx,
-                             hipStream_t s);
+hipError_t sort_pairs_u32(void* tmp, size_t& tmp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                          const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, uint32_t end_bit, hipStream_t s);
+hipError_t launch_fan_draw_keys(const uint32_t* order, const uint32_t* fan_draw, uint32_t n, uint32_t* keys,
+                                hipStream_t s);
+hipError_t launch_fan_gather(const BatchDesc& P, const BatchDesc& M, const uint32_t* order, const uint32_t* draws_of,
+                             const uint32_t* map, uint32_t n_fans, uint32_t* idx, hipStream_t s);
+
+// Uploads P's draws, runs P's pass eagerly (growing the fan capacity as
+// needed) and sizes the sorts and the index triples from the fan counts.
+// *err_out receives P's error bits.
+rf_status sortpass_plan(Device& d, SortPass& S, hipStream_t s, uint64_t* counter, uint32_t* err_out);
+// Uploads the (first position, main draw) table once main_of is set.
+hipError_t sortpass_map(SortPass& S, hipStream_t s);
+// P's pass (reset, culling, vertex shader, clip fans and keys).
+hipError_t sortpass_fans(const SortPass& S, hipStream_t s);
+// The sorts and the gather into M (whose pre draws and scratch are in place).
+hipError_t sortpass_sort_gather(const SortPass& S, const BatchDesc& M, hipStream_t s);
+void sortpass_release(SortPass& S, uint64_t* counter);
+// Device pointer to index triple `first_fan` of the sorted order.
+inline const uint32_t* sortpass_idx(const SortPass& S, uint32_t first_fan) {
+    return (const uint32_t*)S.idx.p + 3ull * first_fan;
+}
 
 // Enqueues the whole pipeline for batch `B` on stream `s` (rf_kernels.hip).
 hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid);

@@ -39,8 +39,7 @@
 namespace rf {
 
 constexpr int WAVE = 64;
-constexpr int TILE_H = (int)TILE_HEIGHT;  // tile rows
-constexpr int BIN_MARGIN = 2;  // extra pixels around a triangle's bounding box
+constexpr int BIN_MARGIN = 2;  // pixels around a triangle's bounding box, besides the drift bound
 // Rows above the target (y < 0) that a triangle may fold onto row 0; more is
 // RF_E_VIEWPORT (the row counter y0 + k, a float in the reference, stays
 // exact below 2^22).
@@ -303,20 +302,39 @@ struct RectAcc {
 
 // Tile rectangle of a triangle's pixels. Rows are exact: tri_fill covers rows
 // floor(top_y + 0.5) .. floor(bot_y + 0.5) (scan, raster.rs:286-300). Columns
-// are the vertex x range widened by BIN_MARGIN pixels (stepped edges may drift
-// by rounding; the rasterizer checks every span stays inside).
+// are the vertex x range widened by a margin that bounds how far the serially
+// stepped edges drift from the exact edges (the rasterizer checks every span
+// stays inside):
+//   - each row step x + dl rounds by at most half an ulp of the values it
+//     passes, so n steps drift by at most n * ulp(X) / 2, X bounding |x|;
+//   - dl itself, (x1 - x0) * (1 / (y1 - y0)), is off by a few ulps relative,
+//     which over the edge's rows adds at most |x1 - x0| * 2^-22.
+// margin_px = BIN_MARGIN + round(D), with D = rows * ulp(X) +
+// (xmax - xmin) * 2^-20 (twice and four times those bounds), tolerates a
+// drift of D + 1 pixel beyond the rounding of the span ends (round_up_to_half)
+// that BIN_MARGIN alone covers. Ordinary content keeps the base margin
+// (1080p: X < 2048, a few hundred rows, D < 0.5); a 16k-row edge at
+// x = 16,000 gets 18 pixels.
+__device__ __forceinline__ int32_t bin_margin(const RectAcc& r, float rows) {
+    const float xmin = __builtin_isfinite(r.xmin) ? r.xmin : 0.0f, xmax = __builtin_isfinite(r.xmax) ? r.xmax : 0.0f;
+    const float X = fmaxf(fabsf(xmin), fabsf(xmax)) + 1.0f;
+    const float ulp = u2f((f2u(X) & 0x7F800000u) - (23u << 23));  // ulp of X (X >= 1)
+    const float D = rows * ulp + (xmax - xmin) * 0x1p-20f;
+    return BIN_MARGIN + (int32_t)floorf(D + 0.5f);
+}
 //
 // A triangle with a NaN vertex (the screen position of a w = 0 clip vertex,
 // 0/0) draws no pixel, but its rows whose left edge is NaN are spans
 // `0..x1` (`x0 as usize` of NaN is 0, raster.rs:104-107) that count in
 // frags.i: its columns start at 0.
-__device__ __forceinline__ bool tile_rect(const RectAcc& r, const FrameDesc& F, uint32_t tw,
+__device__ __forceinline__ bool tile_rect(const RectAcc& r, const FrameDesc& F, uint32_t tw, uint32_t th_log2,
                                           uint32_t* out) {
     const float r0 = floorf(r.ymin + 0.5f), r1 = floorf(r.ymax + 0.5f);
     if (!(r1 > r0)) return false;
     int64_t row0 = (int64_t)r0, row1 = (int64_t)r1 - 1;
     const float xmin = r.nan ? 0.0f : r.xmin, xmax = r.xmax >= xmin ? r.xmax : xmin;
-    int64_t c0 = (int64_t)floorf(xmin) - BIN_MARGIN, c1 = (int64_t)floorf(xmax) + BIN_MARGIN;
+    const int32_t margin = bin_margin(r, r1 - r0);
+    int64_t c0 = (int64_t)floorf(xmin) - margin, c1 = (int64_t)floorf(xmax) + margin;
     // rows above the target fold onto row 0 (`self.y as usize`, raster.rs:106)
     if (row0 < 0) row0 = 0;
     if (row1 < 0) row1 = 0;
@@ -325,7 +343,7 @@ __device__ __forceinline__ bool tile_rect(const RectAcc& r, const FrameDesc& F, 
     if (c1 >= (int64_t)F.width) c1 = (int64_t)F.width - 1;
     if (row1 < row0 || c1 < c0) return false;
     const uint32_t tx0 = (uint32_t)(c0 / tw), tx1 = (uint32_t)(c1 / tw);
-    const uint32_t ty0 = (uint32_t)(row0 / TILE_H), ty1 = (uint32_t)(row1 / TILE_H);
+    const uint32_t ty0 = (uint32_t)row0 >> th_log2, ty1 = (uint32_t)row1 >> th_log2;
     out[0] = tx0 | (ty0 << 16);
     out[1] = tx1 | (ty1 << 16);
     return true;
@@ -360,9 +378,9 @@ __device__ __forceinline__ void for_each_tile(bool valid, uint32_t r0, uint32_t 
 
 // Tile rectangle of an exact pixel rectangle (rows r0..r1, columns c0..c1).
 __device__ __forceinline__ void tile_rect_px(uint32_t r0, uint32_t r1, uint32_t c0, uint32_t c1, uint32_t tw,
-                                             uint32_t* out) {
-    out[0] = (c0 / tw) | ((r0 / TILE_H) << 16);
-    out[1] = (c1 / tw) | ((r1 / TILE_H) << 16);
+                                             uint32_t th_log2, uint32_t* out) {
+    out[0] = (c0 / tw) | ((r0 >> th_log2) << 16);
+    out[1] = (c1 / tw) | ((r1 >> th_log2) << 16);
 }
 
 // One Edge<usize> primitive (prim.rs:41-60): assembly, edge clipping
@@ -481,8 +499,9 @@ __device__ uint32_t edge_setup(const BatchDesc& B, const DrawDesc* D, const Fram
     d[9] = step;
     d[10] = u2f(wide ? 1u : 0u);
     d[11] = 0.0f;
-    if (wide) tile_rect_px((uint32_t)lo, (uint32_t)hi, S, S + n - 1, B.tile_w, rect);
-    else tile_rect_px(S, S + n - 1, (uint32_t)lo, (uint32_t)hi, B.tile_w, rect);
+    const uint32_t th_log2 = (uint32_t)__builtin_ctz(B.tile_h);
+    if (wide) tile_rect_px((uint32_t)lo, (uint32_t)hi, S, S + n - 1, B.tile_w, th_log2, rect);
+    else tile_rect_px(S, S + n - 1, (uint32_t)lo, (uint32_t)hi, B.tile_w, th_log2, rect);
     has_rect = true;
     return 1;
 }
@@ -613,7 +632,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                     const float rr0 = floorf(ra.ymin + 0.5f), rr1 = floorf(ra.ymax + 0.5f);
                     if (rr1 > rr0 && (rr1 > (float)F.height || floorf(ra.xmin) > (float)F.width + 1.0f))
                         atomicOr(B.err, ERRB_BOUNDS);
-                    has_rect = tile_rect(ra, F, B.tile_w, rect);
+                    has_rect = tile_rect(ra, F, B.tile_w, (uint32_t)__builtin_ctz(B.tile_h), rect);
                 }
             }
         }
@@ -1024,7 +1043,7 @@ constexpr int EMPTY_GRAB = RF_EMPTY_GRAB;  // tiles without runs a wave takes at
 
 // The fused clear (Frame::clear) of one tile that nothing draws into: 16-byte
 // stores where whole rows of 4-byte pixels are aligned.
-template <int TW>
+template <int TW, int TILE_H>
 __device__ __forceinline__ void clear_tile(const FrameDesc& F, uint32_t tile, uint32_t lane) {
     const bool has_depth = F.depth != nullptr;
     if (!F.clear_color && !(has_depth && F.clear_depth)) return;
@@ -1076,12 +1095,19 @@ __device__ __forceinline__ void clear_empty_tiles(const BatchDesc& B, uint32_t w
             F = B.frames[fi];
             cur_fi = fi;
         }
-        if (B.tile_w == 16)
-            clear_tile<16>(F, tile, lane);
-        else if (B.tile_w == 32)
-            clear_tile<32>(F, tile, lane);
-        else
-            clear_tile<64>(F, tile, lane);
+        if (B.tile_h == 16) {
+            if (B.tile_w == 16) clear_tile<16, 16>(F, tile, lane);
+            else if (B.tile_w == 32) clear_tile<32, 16>(F, tile, lane);
+            else clear_tile<64, 16>(F, tile, lane);
+        } else if (B.tile_h == 64) {
+            if (B.tile_w == 16) clear_tile<16, 64>(F, tile, lane);
+            else if (B.tile_w == 32) clear_tile<32, 64>(F, tile, lane);
+            else clear_tile<64, 64>(F, tile, lane);
+        } else {
+            if (B.tile_w == 16) clear_tile<16, 32>(F, tile, lane);
+            else if (B.tile_w == 32) clear_tile<32, 32>(F, tile, lane);
+            else clear_tile<64, 32>(F, tile, lane);
+        }
     }
 }
 
@@ -1170,10 +1196,9 @@ struct EdgeRec {
     float lx, rx, lz, l0, l1, l2;
 };
 
-#ifndef RF_REC_CAP
-#define RF_REC_CAP (TILE_H > 32 ? 384 : 128)
-#endif
-constexpr int REC_CAP = RF_REC_CAP;  // edge records per batch
+// Edge records per batch of a tile th rows tall (a batch's first triangle
+// has at most th rows in a tile, unless rows above the target fold onto row 0).
+__host__ __device__ constexpr int rec_cap(int th) { return th > 32 ? 384 : 128; }
 constexpr int FQ_CAP = 512;          // fan queue (ring); one run adds at most 64 * 7 fans
 #ifndef RF_CELL_W
 #define RF_CELL_W 8
@@ -1277,8 +1302,10 @@ __device__ __forceinline__ bool depth_pass_u(int32_t dt, float nz, float cur) {
 // TW: tile width (16, 32, 64); CH: pixels per lane applied together in a span
 // round (6 or 8: short spans waste fewer lanes' slots and registers at 6,
 // long spans take fewer rounds at 8).
-template <int TW, int CH>
+template <int TW, int CH, int TH>
 __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
+    constexpr int TILE_H = TH;  // tile rows
+    constexpr int REC_CAP = rec_cap(TH);
     constexpr int LD = TILE_H + 1;  // padded column stride in LDS (bank spread)
     constexpr int NPX = TW * LD;
     constexpr int CW = CELL_W < TW ? CELL_W : TW;  // ordering cell width in this tile
@@ -1370,7 +1397,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                     E = B.frames[fi];
                     ecur_fi = fi;
                 }
-                clear_tile<TW>(E, tile, lane);
+                clear_tile<TW, TH>(E, tile, lane);
             }
         }
     };
@@ -1420,7 +1447,7 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                          (F.color_stride & 15u) == 0 && (!has_depth || (((uintptr_t)F.depth & 15u) == 0 &&
                                                                          (F.depth_stride & 3u) == 0));
         if (nruns == 0) {
-            clear_tile<TW>(F, tile, lane);
+            clear_tile<TW, TH>(F, tile, lane);
             continue;
         }
 
@@ -2006,25 +2033,41 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
     clear_empties();
 }
 
+// The rasterizer's instances: tile width {16, 32, 64} x span chunk {6, 8}
+// x tile height {16, 32, 64}.
+using RasterFn = void (*)(BatchDesc);
+#define RF_K3(TW, CH) {k_raster<TW, CH, 16>, k_raster<TW, CH, 32>, k_raster<TW, CH, 64>}
+static const RasterFn RASTER_FNS[3][2][3] = {
+    {RF_K3(16, 6), RF_K3(16, 8)}, {RF_K3(32, 6), RF_K3(32, 8)}, {RF_K3(64, 6), RF_K3(64, 8)}};
+#undef RF_K3
+static int raster_index(uint32_t tw, uint32_t ch, uint32_t th, int* i, int* j, int* k) {
+    *i = tw == 16 ? 0 : (tw == 32 ? 1 : 2);
+    *j = ch == 6 ? 0 : 1;
+    *k = th == 16 ? 0 : (th == 64 ? 2 : 1);
+    return 0;
+}
+
 hipError_t launch_raster(const BatchDesc& B, hipStream_t s, uint32_t raster_grid) {
-    const bool c6 = B.chunk == 6;
-    if (B.tile_w == 16) {
-        if (c6)
-            hipLaunchKernelGGL((k_raster<16, 6>), dim3(raster_grid), dim3(64), 0, s, B);
-        else
-            hipLaunchKernelGGL((k_raster<16, 8>), dim3(raster_grid), dim3(64), 0, s, B);
-    } else if (B.tile_w == 32) {
-        if (c6)
-            hipLaunchKernelGGL((k_raster<32, 6>), dim3(raster_grid), dim3(64), 0, s, B);
-        else
-            hipLaunchKernelGGL((k_raster<32, 8>), dim3(raster_grid), dim3(64), 0, s, B);
-    } else {
-        if (c6)
-            hipLaunchKernelGGL((k_raster<64, 6>), dim3(raster_grid), dim3(64), 0, s, B);
-        else
-            hipLaunchKernelGGL((k_raster<64, 8>), dim3(raster_grid), dim3(64), 0, s, B);
-    }
+    int i, j, k;
+    raster_index(B.tile_w, B.chunk, B.tile_h, &i, &j, &k);
+    hipLaunchKernelGGL(RASTER_FNS[i][j][k], dim3(raster_grid), dim3(64), 0, s, B);
     return hipGetLastError();
+}
+
+uint32_t raster_wg_per_cu(uint32_t tile_w, uint32_t chunk, uint32_t tile_h) {
+    // resident one-wave workgroups per CU of each instance (registers and
+    // LDS decide), asked of the runtime once
+    static int occ[3][2][3] = {};
+    int i, j, k;
+    raster_index(tile_w, chunk, tile_h, &i, &j, &k);
+    int& n = occ[i][j][k];
+    if (n == 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, RASTER_FNS[i][j][k], 64, 0) != hipSuccess || v <= 0)
+            v = 4;
+        n = v;
+    }
+    return (uint32_t)n;
 }
 #endif  // RF_RASTER_TU
 
@@ -2037,23 +2080,31 @@ hipError_t launch_raster(const BatchDesc& B, hipStream_t s, uint32_t raster_grid
 //                 Render::depth, (a.z + b.z + c.z) / 3 (prim.rs:21-23), as a
 //                 total_cmp order key, then the triangle and fan index, so
 //                 equal depths keep submission order;
-//   radix sort of (key, fan) pairs;
-//   k_fan_gather  fan i of the sorted order becomes vertex slots 3i..3i+2
-//                 (clip position, attributes, outcode 0, screen vertex): the
-//                 main pipeline then draws them as unclipped triangles.
+//   radix sort of (key, fan) pairs; with several sorted draws, a second,
+//                 stable sort by draw, so each draw's fans are contiguous and
+//                 in order (each render() call sorts its own primitives);
+//   k_fan_gather  fan i of the sorted order becomes vertex slots
+//                 3i'..3i'+2 of its draw, i' its rank in the draw (clip
+//                 position, attributes, outcode 0, screen vertex): the main
+//                 pipeline then draws them as unclipped triangles.
+// The sorted draws form a batch of their own (their original meshes); the
+// gather writes into the main batch, whose descriptors of these draws hold
+// the fan counts the planning run found.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void put_fan(const BatchDesc& B, const CV& a, const CV& b, const CV& c, uint64_t g,
-                                        uint32_t j) {
+__device__ __forceinline__ void put_fan(const BatchDesc& B, const DrawDesc* D, uint32_t di, const CV& a,
+                                        const CV& b, const CV& c, uint64_t g, uint32_t j) {
     const uint32_t f = atomicAdd(B.fan_count, 1u);
+    if (B.fan_cnt) atomicAdd(&B.fan_cnt[di], 1u);
     if (f >= B.fan_cap) {
         atomicOr(B.err, ERRB_EXTRA);
         return;
     }
     const float z = ((a.p[2] + b.p[2]) + c.p[2]) / 3.0f;
     uint32_t k = (uint32_t)total_key(z) ^ 0x80000000u;  // ascending total order as unsigned
-    if (B.depth_sort == 2) k = ~k;                      // back to front
+    if (D->sort == 2) k = ~k;                           // back to front
     B.fan_keys[f] = ((unsigned long long)k << 32) | (g << 3) | j;
     B.fan_vals[f] = f;
+    if (B.fan_draw) B.fan_draw[f] = di;
     float* o = B.fans + (uint64_t)f * FAN_FLOATS;
     const CV* v[3] = {&a, &b, &c};
     for (int q = 0; q < 3; ++q) {
@@ -2065,14 +2116,17 @@ __device__ __forceinline__ void put_fan(const BatchDesc& B, const CV& a, const C
 __global__ __launch_bounds__(256) void k_fans(BatchDesc B) {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= B.n_tris) return;
-    const DrawDesc* D = &B.draws[0];
-    const uint32_t i0 = D->tris[3 * g], i1 = D->tris[3 * g + 1], i2 = D->tris[3 * g + 2];
+    const uint32_t di = draw_of_tri(B.draws, B.n_draws, B.group_draw[g / WAVE], g);
+    if (B.any_cull && !B.draw_vis[di]) return;  // a draw the bounding-box test hid
+    const DrawDesc* D = &B.draws[di];
+    const uint64_t t = g - D->tri_base;
+    const uint32_t i0 = D->tris[3 * t], i1 = D->tris[3 * t + 1], i2 = D->tris[3 * t + 2];
     if (i0 >= D->n_verts || i1 >= D->n_verts || i2 >= D->n_verts) {
         atomicOr(B.err, ERRB_INDEX);
         return;
     }
     CV poly[CLIP_MAX], tmp[CLIP_MAX];
-    const uint32_t ix[3] = {i0, i1, i2};
+    const uint64_t ix[3] = {D->vert_base + i0, D->vert_base + i1, D->vert_base + i2};
     for (int k = 0; k < 3; ++k) {
         const float4 p = B.cv[ix[k]], a = B.ca[ix[k]];
         const float pp[4] = {p.x, p.y, p.z, p.w}, aa[3] = {a.x, a.y, a.z};
@@ -2081,30 +2135,43 @@ __global__ __launch_bounds__(256) void k_fans(BatchDesc B) {
     const uint32_t all_out = poly[0].oc & poly[1].oc & poly[2].oc, any_out = poly[0].oc | poly[1].oc | poly[2].oc;
     if (all_out != 0) return;  // Hidden
     if (any_out == 0) {        // Visible
-        put_fan(B, poly[0], poly[1], poly[2], g, 0);
+        put_fan(B, D, di, poly[0], poly[1], poly[2], g, 0);
         return;
     }
     const int n = clip_polygon(poly, 3, tmp);
     if (n < 0) atomicOr(B.err, ERRB_CLIP);
-    for (int k = 1; k + 1 < n; ++k) put_fan(B, poly[0], poly[k], poly[k + 1], g, (uint32_t)(k - 1));
+    for (int k = 1; k + 1 < n; ++k) put_fan(B, D, di, poly[0], poly[k], poly[k + 1], g, (uint32_t)(k - 1));
 }
 
-__global__ __launch_bounds__(256) void k_fan_gather(BatchDesc B, const uint32_t* order, uint32_t n_fans,
+// The second sort's keys: the draw of each fan in the first sort's order.
+__global__ __launch_bounds__(256) void k_fan_draw_keys(const uint32_t* order, const uint32_t* fan_draw, uint32_t n,
+                                                       uint32_t* keys) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = fan_draw[order[i]];
+}
+
+// Fan i of the final order (its draw d = draws_of[i], 0 if null) becomes
+// vertex slots of main draw map[2d + 1] at rank i - map[2d] (the draw's first
+// position) and index triples idx[3i..3i+2] (relative to that draw).
+__global__ __launch_bounds__(256) void k_fan_gather(BatchDesc P, BatchDesc M, const uint32_t* order,
+                                                    const uint32_t* draws_of, const uint32_t* map, uint32_t n_fans,
                                                     uint32_t* idx) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_fans) return;
-    const DrawDesc* D = &B.draws[0];
-    const float* src = B.fans + (uint64_t)order[i] * FAN_FLOATS;
+    const uint32_t d = draws_of ? draws_of[i] : 0u;
+    const uint32_t rank = i - map[2 * d];
+    const DrawDesc* D = &M.draws[map[2 * d + 1]];
+    const float* src = P.fans + (uint64_t)order[i] * FAN_FLOATS;
     const bool persp = kind_persp(D->kind);
     for (int q = 0; q < 3; ++q) {
-        const uint64_t slot = 3ull * i + q;
+        const uint64_t slot = D->vert_base + 3ull * rank + q;
         CV c;
         for (int e = 0; e < 4; ++e) c.p[e] = src[7 * q + e];
         for (int e = 0; e < 3; ++e) c.a[e] = src[7 * q + 4 + e];
-        B.cv[slot] = make_float4(c.p[0], c.p[1], c.p[2], c.p[3]);
-        B.ca[slot] = make_float4(c.a[0], c.a[1], c.a[2], u2f(0u));  // inside: clipped already
-        to_screen(c, D->vp, persp, B.sv + 6 * slot);
-        idx[slot] = (uint32_t)slot;
+        M.cv[slot] = make_float4(c.p[0], c.p[1], c.p[2], c.p[3]);
+        M.ca[slot] = make_float4(c.a[0], c.a[1], c.a[2], u2f(0u));  // inside: clipped already
+        to_screen(c, D->vp, persp, M.sv + 6 * slot);
+        idx[3ull * i + q] = 3u * rank + (uint32_t)q;
     }
 }
 
@@ -2135,8 +2202,9 @@ __global__ __launch_bounds__(256) void k_cull(BatchDesc B) {
             vis = 0;
             FrameStats& st = B.stats[D->frame];
             atomicAdd(&st.hidden_draws, 1ull);
-            atomicAdd(&st.hidden_prims, (unsigned long long)D->n_tris);
-            atomicAdd(&st.hidden_verts, (unsigned long long)D->n_verts);
+            // a depth-sorted draw holds its fans: its render() call's own counts
+            atomicAdd(&st.hidden_prims, (unsigned long long)(D->pre ? D->in_tris : D->n_tris));
+            atomicAdd(&st.hidden_verts, (unsigned long long)(D->pre ? D->in_verts : D->n_verts));
         }
     }
     B.draw_vis[i] = vis;
@@ -2165,6 +2233,7 @@ __global__ void k_reset(BatchDesc B) {
         if (B.fan_count) *B.fan_count = 0;
         for (uint32_t k = 0; k < TILE_PARTS; ++k) B.tile_counter[k] = 0;
     }
+    if (B.fan_cnt && i < B.n_draws) B.fan_cnt[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2218,17 +2287,26 @@ hipError_t launch_fans(const BatchDesc& B, hipStream_t s) {
     if (B.n_tris && (!B.fans || !B.fan_keys || !B.fan_vals || !B.fan_count || !B.fan_cap)) return hipErrorInvalidValue;
     uint32_t reset_n = B.n_tiles > B.n_frames * PRIM_PARTS ? B.n_tiles : B.n_frames * PRIM_PARTS;
     reset_n = reset_n > B.n_groups ? reset_n : B.n_groups;
+    reset_n = reset_n > B.n_draws ? reset_n : B.n_draws;
     const uint32_t n_vblocks = (uint32_t)((B.n_verts + 255) / 256);
     reset_n = reset_n > n_vblocks ? reset_n : n_vblocks;
     hipLaunchKernelGGL(k_reset, dim3((reset_n + 255) / 256 + 1), dim3(256), 0, s, B);
+    if (B.any_cull) hipLaunchKernelGGL(k_cull, dim3((B.n_draws + 255) / 256), dim3(256), 0, s, B);
     if (B.n_verts) hipLaunchKernelGGL(k_vertex, dim3(n_vblocks), dim3(256), 0, s, B);
     if (B.n_tris) hipLaunchKernelGGL(k_fans, dim3((uint32_t)((B.n_tris + 255) / 256)), dim3(256), 0, s, B);
     return hipGetLastError();
 }
-hipError_t launch_fan_gather(const BatchDesc& B, const uint32_t* order, uint32_t n_fans, uint32_t* idx,
-                             hipStream_t s) {
-    if (n_fans && (!B.fans || !order || !idx || !B.cv || !B.ca || !B.sv)) return hipErrorInvalidValue;
-    if (n_fans) hipLaunchKernelGGL(k_fan_gather, dim3((n_fans + 255) / 256), dim3(256), 0, s, B, order, n_fans, idx);
+hipError_t launch_fan_draw_keys(const uint32_t* order, const uint32_t* fan_draw, uint32_t n, uint32_t* keys,
+                                hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fan_draw_keys, dim3((n + 255) / 256), dim3(256), 0, s, order, fan_draw, n, keys);
+    return hipGetLastError();
+}
+hipError_t launch_fan_gather(const BatchDesc& P, const BatchDesc& M, const uint32_t* order, const uint32_t* draws_of,
+                             const uint32_t* map, uint32_t n_fans, uint32_t* idx, hipStream_t s) {
+    if (n_fans && (!P.fans || !order || !map || !idx || !M.cv || !M.ca || !M.sv)) return hipErrorInvalidValue;
+    if (n_fans)
+        hipLaunchKernelGGL(k_fan_gather, dim3((n_fans + 255) / 256), dim3(256), 0, s, P, M, order, draws_of, map,
+                           n_fans, idx);
     return hipGetLastError();
 }
 hipError_t launch_pipeline(const BatchDesc& B, hipStream_t s, uint32_t raster_grid) {

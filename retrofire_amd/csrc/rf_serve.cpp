@@ -123,7 +123,8 @@ struct Plan {
         n_tris += q.n_tris;
         n_draws += q.draws.size();
         n_frames += 1;
-        n_tiles += (uint64_t)((q.r.width + SERVE_TILE_W - 1) / SERVE_TILE_W) * ((q.r.height + TILE_ROWS - 1) / TILE_ROWS);
+        n_tiles += (uint64_t)((q.r.width + SERVE_TILE_W - 1) / SERVE_TILE_W) *
+                   ((q.r.height + DEFAULT_TILE_H - 1) / DEFAULT_TILE_H);
         const uint64_t px = (uint64_t)q.r.width * q.r.height;
         targets += align_up(px * bytes_per_pixel(q.r.fmt)) + (q.r.with_depth ? align_up(px * 4) : 0);
     }
@@ -346,7 +347,7 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     B.n_groups = (uint32_t)((p.n_tris + WAVE_SIZE - 1) / WAVE_SIZE);
     B.n_tiles = (uint32_t)p.n_tiles;
     B.tile_w = SERVE_TILE_W;
-    B.tile_h = TILE_ROWS;
+    B.tile_h = DEFAULT_TILE_H;
     B.chunk = RF_SERVE_CHUNK;  // span chunk of served launches
     B.sort_tpw = 1;            // few frames a launch: one tile per binsort wave
     B.extra_cap = l.extra_cap;
@@ -381,7 +382,7 @@ static rf_status launch_lane(Serve& s, Lane& l) {
         c.clear_color = 1;  // a served frame starts from Frame::clear
         c.clear_depth = 1;
         fill_frame_ctx(f, c);
-        set_grid(f, SERVE_TILE_W, tile_base);
+        set_grid(f, SERVE_TILE_W, DEFAULT_TILE_H, tile_base);
         fds.push_back(f);
         for (const rf_mesh_draw& md : q.draws) {
             const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
@@ -441,7 +442,7 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     // Nothing is queued before this point; from here on a failure leaves
     // work on the stream, which the caller drains before the block is reused.
     if (hipMemcpyAsync(blk, l.h_desc, total, hipMemcpyHostToDevice, l.stream)) return RF_E_DEVICE;
-    if (launch_pipeline(B, l.stream, raster_grid(d, B.n_tiles))) return RF_E_DEVICE;
+    if (launch_pipeline(B, l.stream, raster_grid(d, B))) return RF_E_DEVICE;
     if (hipMemcpyAsync(l.h_stats, B.stats, fds.size() * sizeof(FrameStats), hipMemcpyDeviceToHost, l.stream) ||
         hipMemcpyAsync(l.h_misc, misc_words(B), 12, hipMemcpyDeviceToHost, l.stream) ||
         hipMemcpyAsync(l.h_misc + 4, B.run_off + B.n_tiles, 4, hipMemcpyDeviceToHost, l.stream))

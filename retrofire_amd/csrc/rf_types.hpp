@@ -14,11 +14,6 @@
 
 namespace rf {
 
-#ifndef RF_TILE_H
-#define RF_TILE_H 32
-#endif
-// Rasterizer tile height in rows (the binning grid and the rasterizer agree on it).
-constexpr uint32_t TILE_HEIGHT = RF_TILE_H;
 // The rasterizer's tiles are split into this many contiguous partitions, one
 // per XCD (accelerator complex die, each with its own L2) of the MI355X.
 constexpr uint32_t TILE_PARTS = 8;
@@ -94,6 +89,8 @@ struct DrawDesc {
     uint32_t cull;                      // 1: skipped when BBox::visibility(mvp) is Hidden (scene.rs:81-87)
     uint32_t prim;                      // PrimType; n_tris counts primitives of that type
     uint32_t pre;                       // 1: vertex slots already filled (depth-sorted clip fans)
+    uint32_t sort;                      // depth sort of this render() call: 0 none, 1 front to back, 2 back to front
+    uint32_t in_tris, in_verts;         // a pre draw's render() input counts (prims.i, verts.i), for culled draws
     uint64_t vert_base;                 // first slot in the batch vertex buffers
     uint64_t tri_base;                  // first global triangle index
     float mvp[16];                      // row-major ProjMat3 (the uniform)
@@ -153,7 +150,7 @@ struct BatchDesc {
     uint64_t n_tris;    // total input triangles
     uint32_t n_groups;  // ceil(n_tris / 64)
     uint32_t n_tiles;   // total tiles over all frames
-    uint32_t tile_w, tile_h;
+    uint32_t tile_w, tile_h;  // rasterizer tile: 16, 32 or 64 columns x 16, 32 or 64 rows
     uint32_t chunk;     // rasterizer span chunk: 6 or 8 pixels (anything else: 8)
     uint32_t sort_tpw;  // tiles per k_binsort wave (1..64; 0: the default, 8)
     // scratch
@@ -190,6 +187,8 @@ struct BatchDesc {
     unsigned long long* fan_keys;     // per fan: order key << 32 | triangle << 3 | fan
     uint32_t* fan_vals;               // per fan: its index in `fans`
     uint32_t* fan_count;
+    uint32_t* fan_draw;               // per fan: its draw
+    uint32_t* fan_cnt;                // per draw: its fans
 };
 constexpr int FAN_FLOATS = 21;
 

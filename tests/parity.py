@@ -6,6 +6,26 @@ import oracle_lib as O
 import retrofire_amd as rf
 
 DT = {None: 0, "less": 1, "equal": 2, "greater": 3}
+
+# Rasterizer configurations (tile width, span chunk, tile height) every
+# parity case runs with: each width with each chunk at the default height,
+# and the other heights (results must be identical for all of them).
+TILE_CONFIGS = [(16, 8, 32), (32, 8, 32), (64, 8, 32), (16, 6, 32), (32, 6, 32), (64, 6, 32),
+                (32, 8, 16), (32, 8, 64), (16, 6, 64), (64, 6, 16)]
+TILE_IDS = [f"tw{w}c{c}h{h}" for (w, c, h) in TILE_CONFIGS]
+
+
+def use_tile_config(cfg):
+    """Sets the device's rasterizer configuration for rf_render, then restores
+    the defaults (a generator for a pytest fixture)."""
+    dev = rf.default_device()
+    dev.set_tile_width(cfg[0])
+    dev.set_raster_chunk(cfg[1])
+    dev.set_tile_height(cfg[2])
+    yield cfg
+    dev.set_tile_width(0)
+    dev.set_raster_chunk(0)
+    dev.set_tile_height(0)
 FC = {None: 0, "back": 1, "front": 2}
 DS = {None: 0, "front_to_back": 1, "back_to_front": 2}
 

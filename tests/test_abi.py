@@ -42,8 +42,9 @@ def test_struct_layouts_match_header():
 #include <stdio.h>
 #include "retrofire_amd.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rf_ctx), sizeof(rf_stats), sizeof(rf_target),
-         sizeof(rf_draw), sizeof(rf_mesh_draw), sizeof(rf_frame), sizeof(rf_request), sizeof(rf_result));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rf_ctx), sizeof(rf_stats), sizeof(rf_target),
+         sizeof(rf_draw), sizeof(rf_mesh_draw), sizeof(rf_frame), sizeof(rf_request), sizeof(rf_result),
+         sizeof(rf_tune_entry));
   return 0;
 }'''
     with tempfile.TemporaryDirectory() as d:
@@ -53,7 +54,7 @@ int main(void) {
         subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
         sizes = [int(x) for x in subprocess.check_output([exe]).split()]
     ours = [C.sizeof(t) for t in (_ffi.RfCtx, _ffi.RfStats, _ffi.RfTarget, _ffi.RfDraw, _ffi.RfMeshDraw,
-                                  _ffi.RfFrame, _ffi.RfRequest, _ffi.RfResult)]
+                                  _ffi.RfFrame, _ffi.RfRequest, _ffi.RfResult, _ffi.RfTuneEntry)]
     assert ours == sizes
 
 

@@ -75,24 +75,32 @@ def test_batched_uneven_frames_match_oracle(world):
                  RB.mesh_draw(icos, rf.shader.Gouraud(), f["mvp_i"], f["vp"], n_tris=f["ni"])]
         spec.append((fb, rf.Context(clear=True), draws))
     dev = rf.default_device()
-    # autotuned, then every forced tile width with each span chunk
-    for tw, ch in ((0, 0), (16, 8), (32, 8), (64, 8), (16, 6), (32, 6), (64, 6)):
+    oracle = [oracle_frame(world, f) for f in frames]
+    # autotuned, then every forced tile width with each span chunk, and other tile heights
+    for tw, ch, th in ((0, 0, 0), (16, 8, 32), (32, 8, 32), (64, 8, 32), (16, 6, 32), (32, 6, 32), (64, 6, 32),
+                       (32, 8, 16), (64, 6, 64), (16, 8, 64)):
         dev.set_raster_chunk(ch)
+        dev.set_tile_height(th)
         b = RB.Batch(spec, tile_w=tw)
         assert b.chunk in (6, 8) and (tw == 0 or b.chunk == ch)
+        assert b.tile_h in (16, 32, 64) and (tw == 0 or b.tile_h == th)
+        if tw == 0:
+            assert len(b.tune_results()) == 18
         for rep in range(3):  # graph replays give the same result every time
             b.run(sync=True)
             stats = b.wait()
             for k, f in enumerate(frames):
                 col, dep = fbs[k].read()
-                ocol, odep, ost = oracle_frame(world, f)
-                assert np.array_equal(col, ocol), f"frame {k} colour (tile_w {b.tile_w}, chunk {b.chunk}, replay {rep})"
+                ocol, odep, ost = oracle[k]
+                assert np.array_equal(col, ocol), \
+                    f"frame {k} colour (tile {b.tile_w}x{b.tile_h}, chunk {b.chunk}, replay {rep})"
                 assert np.array_equal(dep.view(np.uint32), odep.view(np.uint32)), f"frame {k} depth"
                 s = stats[k]
                 assert [s.prims_o, s.verts_o, s.frags_i, s.frags_o] == [int(x) for x in ost[[2, 4, 5, 6]]]
                 assert s.calls == 2
         b.free()
     dev.set_raster_chunk(0)
+    dev.set_tile_height(0)
     for fb in fbs:
         fb.free()
     crates.free()
@@ -258,3 +266,77 @@ def test_argument_checks():
     assert e.value.status == -3
     fb.free()
     m.free()
+
+
+@pytest.mark.parametrize("sort", ["front_to_back", "back_to_front"])
+@pytest.mark.parametrize("dt", [None, "less"])
+def test_depth_sorted_frames_in_batches(sort, dt):
+    """Batched frames whose context asks for a depth sort (render.rs:180-183,
+    209-219): every render() call sorts its own clip fans (near-plane fans
+    included), each frame holds several sorted draws of every shader kind,
+    edges keep their order, frames without a sort sit between them, and some
+    draws are culled whole; every frame, replayed, equals the oracle."""
+    from test_gpu_parity import random_soup
+    rng = np.random.default_rng(70 + (sort == "back_to_front") + 2 * (dt is None))
+    tex = rng.integers(0, 256, size=(32, 64, 4), dtype=np.uint8)
+    dtex = RB.Texture(tex)
+    shaders = {0: rf.shader.Flat((9, 200, 77, 255)), 1: rf.shader.Gouraud(), 2: rf.shader.TexClamp(tex),
+               3: rf.shader.TexRepeatPot(tex)}
+    meshes, frames, oracle = [], [], []
+    for fi in range(6):
+        w, h = [(301, 197), (256, 256), (128, 96)][fi % 3]
+        sorted_frame = fi != 2
+        ctx = rf.Context(clear=True, depth_test=dt, face_cull=None, depth_sort=sort if sorted_frame else None)
+        fb = RB.DeviceFramebuf(w, h, 0, True)
+        ocol = np.zeros((h, w, 4), np.uint8)
+        ocol[...] = (0, 0, 0, 255)
+        odep = np.zeros((h, w), np.float32)
+        ost = np.zeros(8, np.uint64)
+        draws = []
+        for k, kind in enumerate([1, 3, 0, 2, 1]):
+            pos, attr, tris, mvp, vp = random_soup(rng, 60 + 30 * k, w, h, kind)
+            if k == 4:  # far off to the side: culled whole by its bounding box
+                mvp = M.then(M.translate3(40.0, 0.0, 0.0), mvp)
+            mesh = RB.Mesh(pos, tris, attr)
+            meshes.append(mesh)
+            draws.append(RB.mesh_draw(mesh, shaders[kind], mvp, vp, texture=dtex if kind >= 2 else None,
+                                      cull=(k == 4)))
+            if not (k == 4 and O.bbox_visibility(O.bbox_of(pos), mvp) == O.HIDDEN):
+                e, _ = O.render(kind, tris, pos, attr, mvp, vp, ocol, depth=odep, flat=(9, 200, 77, 255),
+                                tex=tex if kind >= 2 else None,
+                                ctx=(DT_[dt], 0, 1, 1, DS_[sort] if sorted_frame else 0), stats=ost)
+                assert e == 0
+        # an edge draw keeps its order under the sort
+        epos = rng.uniform(-1, 1, size=(80, 3)).astype(np.float32)
+        ecol = rng.uniform(0, 1, size=(80, 3)).astype(np.float32)
+        edges = np.arange(80, dtype=np.uint32).reshape(40, 2)
+        emesh = RB.Mesh(epos, edges, ecol)
+        meshes.append(emesh)
+        draws.append(RB.mesh_draw(emesh, rf.shader.Gouraud(), M.identity(), M.viewport(0, h, w, 0)))
+        e, _ = O.render(1, edges, epos, ecol, M.identity(), M.viewport(0, h, w, 0), ocol, depth=odep,
+                        ctx=(DT_[dt], 0, 1, 1, DS_[sort] if sorted_frame else 0), stats=ost, prim=1)
+        assert e == 0
+        frames.append((fb, ctx, draws))
+        oracle.append((ocol, odep, ost))
+    b = RB.Batch(frames)
+    for rep in range(3):
+        b.run(sync=True)
+        stats = b.wait()
+        for k, (fb, _, _) in enumerate(frames):
+            ocol, odep, ost = oracle[k]
+            gcol, gdep = fb.read()
+            assert np.array_equal(gcol, ocol), f"frame {k} colour (replay {rep})"
+            assert np.array_equal(gdep.view(np.uint32), odep.view(np.uint32)), f"frame {k} depth"
+            s = stats[k]
+            assert [s.calls, s.prims_i, s.prims_o, s.verts_i, s.verts_o, s.frags_i, s.frags_o] == \
+                [int(x) for x in ost[:7]], f"frame {k} stats"
+    b.free()
+    for fb, _, _ in frames:
+        fb.free()
+    for m in meshes:
+        m.free()
+    dtex.free()
+
+
+DT_ = {None: 0, "less": 1, "equal": 2, "greater": 3}
+DS_ = {None: 0, "front_to_back": 1, "back_to_front": 2}

@@ -7,7 +7,7 @@ import pytest
 
 import oracle_lib as O
 import retrofire_amd as rf
-from parity import assert_same, run_both
+from parity import assert_same, run_both, use_tile_config
 from retrofire_amd import batch as RB
 from retrofire_amd import mat as M
 from retrofire_amd import scenes as S
@@ -15,12 +15,10 @@ from retrofire_amd import scenes as S
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=[16, 32, 64], ids=["tw16", "tw32", "tw64"])
-def tile_width(request):
-    dev = rf.default_device()
-    dev.set_tile_width(request.param)
-    yield request.param
-    dev.set_tile_width(0)
+@pytest.fixture(autouse=True, params=[(16, 8, 32), (32, 8, 32), (64, 8, 32), (32, 6, 16), (16, 8, 64)],
+                ids=["tw16", "tw32", "tw64", "tw32c6h16", "tw16h64"])
+def tile_config(request):
+    yield from use_tile_config(request.param)
 
 
 def edge_soup(rng, n, w, h, kind, spread=3.0):

@@ -8,7 +8,7 @@ import pytest
 
 import oracle_lib as O
 import retrofire_amd as rf
-from parity import assert_same, run_both
+from parity import TILE_CONFIGS, TILE_IDS, assert_same, run_both, use_tile_config
 from retrofire_amd import batch as RB
 from retrofire_amd import mat as M
 from retrofire_amd import scenes as S
@@ -17,16 +17,10 @@ from test_oracle_kats import GOLDEN, hello_tri_inputs, read_ppm, textured_quad_i
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=[(16, 8), (32, 8), (64, 8), (16, 6), (32, 6), (64, 6)],
-                ids=["tw16", "tw32", "tw64", "tw16c6", "tw32c6", "tw64c6"])
-def tile_width(request):
-    """Every case runs with each rasterizer tile width and span chunk."""
-    dev = rf.default_device()
-    dev.set_tile_width(request.param[0])
-    dev.set_raster_chunk(request.param[1])
-    yield request.param[0]
-    dev.set_tile_width(0)
-    dev.set_raster_chunk(0)
+@pytest.fixture(autouse=True, params=TILE_CONFIGS, ids=TILE_IDS)
+def tile_config(request):
+    """Every case runs with each rasterizer configuration."""
+    yield from use_tile_config(request.param)
 
 
 def test_textured_quad_golden_gpu():  # core/tests/rendering.rs:18-60
@@ -291,14 +285,16 @@ def test_depth_sort_crates_and_edges():
     assert_same(o, g)
 
 
-def test_depth_sort_rejected_by_batches():
+def test_depth_sort_rejected_by_the_serving_loop():
     m = RB.Mesh(np.zeros((3, 3), np.float32), np.array([[0, 1, 2]], np.uint32))
-    fb = RB.DeviceFramebuf(16, 16)
-    with pytest.raises(rf.RenderError) as e:
-        RB.Batch([(fb, rf.Context(depth_sort="front_to_back"),
-                         [RB.mesh_draw(m, rf.shader.Flat(), M.identity(), M.viewport(0, 16, 16, 0))])])
-    assert e.value.status == -5
-    fb.free()
+    srv = RB.Server(budget_bytes=1 << 24, max_batch=2, n_streams=1)
+    try:
+        with pytest.raises(rf.RenderError) as e:
+            srv.submit(1, [RB.mesh_draw(m, rf.shader.Flat(), M.identity(), M.viewport(0, 16, 16, 0))], 16, 16,
+                       ctx=rf.Context(depth_sort="front_to_back"))
+        assert e.value.status == -5
+    finally:
+        srv.close()
     m.free()
 
 
@@ -383,3 +379,48 @@ def test_w_zero_vertex():
         o, g = run_both(kind, tris, pos, attr, mvp, vp, w, h, tex=tex, ctx=rf.Context(clear=True, face_cull=None))
         assert_same(o, g)
         assert g[2][5] > g[2][6]  # the NaN rows count in frags.i without writing
+
+
+_DRIFT = {}
+
+
+def _drift_case():
+    """Thin triangles on a 16384 x 16384 target whose long, nearly vertical
+    edges at x ~ 16,000 step by just over half an ulp of x per row: every row
+    step rounds up by one ulp, so after 16k rows the stepped edge runs pixels
+    past its end vertex (raster.rs:84-113)."""
+    if "o" not in _DRIFT:
+        w = h = 16384
+        verts, tris = [], []
+        ulp = 2.0 ** -10  # x in [8192, 16384)
+        for i, (x, slope) in enumerate([(16000.25, 0.51), (15800.75, -0.52), (9000.5, 0.55), (12000.1, -0.6)]):
+            dl = slope * ulp
+            top = (x, 0.3)
+            bot = (x + dl * 16383.0, 16383.4)
+            side = (x - 30.0 if slope > 0 else x + 30.0, 9000.2)
+            verts += [top, bot, side]
+            tris.append([3 * i, 3 * i + 1, 3 * i + 2])
+        # screen coordinates through an orthographic mvp: x, y in pixels
+        pos = np.array([(x, y, 0.5) for (x, y) in verts], np.float32)
+        mvp = M.then(M.scale3(2.0 / w, 2.0 / h, 1.0), M.translate3(-1.0, -1.0, 0.0))
+        vp = M.viewport(0, 0, w, h)
+        tris = np.array(tris, np.uint32)
+        cols = np.random.default_rng(9).uniform(0, 1, size=(pos.shape[0], 3)).astype(np.float32)
+        ocol = np.zeros((h, w, 4), np.uint8)
+        ocol[...] = (0, 0, 0, 255)
+        ost = np.zeros(8, np.uint64)
+        err, _ = O.render(1, tris, pos, cols, mvp, vp, ocol, ctx=(0, 0, 1, 1), stats=ost)
+        assert err == 0
+        _DRIFT.update(o=(ocol, ost), inputs=(tris, pos, cols, mvp, vp, w, h))
+    return _DRIFT
+
+
+def test_long_edges_drift_past_their_vertices():
+    d = _drift_case()
+    tris, pos, cols, mvp, vp, w, h = d["inputs"]
+    ocol, ost = d["o"]
+    gcol = np.zeros((h, w, 4), np.uint8)
+    ctx = rf.Context(clear=True, face_cull=None, depth_test=None)
+    rf.render(tris, (pos, cols), rf.shader.Gouraud(), mvp, vp, rf.Colorbuf(gcol), ctx)
+    assert np.array_equal(gcol, ocol)
+    assert (ctx.stats.frags_i, ctx.stats.frags_o) == (int(ost[5]), int(ost[6])) and ctx.stats.frags_o > 100000

@@ -6,7 +6,7 @@ import pytest
 
 import oracle_lib as O
 import retrofire_amd as rf
-from parity import assert_same, run_both
+from parity import TILE_CONFIGS, TILE_IDS, assert_same, run_both, use_tile_config
 from retrofire_amd import batch as RB
 from retrofire_amd import mat as M
 from test_gpu_parity import random_soup
@@ -14,16 +14,10 @@ from test_gpu_parity import random_soup
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=[(16, 8), (32, 8), (64, 8), (16, 6), (32, 6), (64, 6)],
-                ids=["tw16", "tw32", "tw64", "tw16c6", "tw32c6", "tw64c6"])
-def tile_width(request):
-    """Every case runs with each rasterizer tile width and span chunk."""
-    dev = rf.default_device()
-    dev.set_tile_width(request.param[0])
-    dev.set_raster_chunk(request.param[1])
-    yield request.param[0]
-    dev.set_tile_width(0)
-    dev.set_raster_chunk(0)
+@pytest.fixture(autouse=True, params=TILE_CONFIGS, ids=TILE_IDS)
+def tile_config(request):
+    """Every case runs with each rasterizer configuration."""
+    yield from use_tile_config(request.param)
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
