@@ -193,6 +193,21 @@ rf_status rf_device_set_tile_height(rf_device* dev, uint32_t tile_h);
 rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, const rf_ctx* ctx,
                     rf_stats* stats);
 
+/* rf_render without device allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and errors as rf_render, but every device buffer (the
+ * draw's vertices, indices and texels, the target, descriptors and pipeline
+ * scratch) comes from one static device array of 64 MiB per GPU, held from
+ * the first rf_device_open until the process ends; the call makes no
+ * hipMalloc or hipFree. A draw whose buffers do not fit the array returns
+ * RF_E_BUDGET before any device work (a 1920x1080 RGBA8 target with depth
+ * takes 16.6 MB of it); ctx->depth_sort returns RF_E_UNSUPPORTED (its sort
+ * needs temporary storage: use rf_render). Calls on one GPU share the
+ * array: calls that overlap in time on one GPU (from several threads) race. */
+rf_status rf_render_noalloc(rf_device* dev, const rf_draw* draw, rf_target* target, const rf_ctx* ctx,
+                            rf_stats* stats);
+
 /* ---- residency ---------------------------------------------------------
  * Device-resident meshes and textures (geom/mesh.rs:24-30, tex.rs:33-37).
  * n_attr is the attribute float count of the kinds the mesh is drawn with. */

@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _ffi
 
-__all__ = ["Context", "Stats", "Framebuf", "Colorbuf", "RenderError", "render", "shader", "Device",
+__all__ = ["Context", "Stats", "Framebuf", "Colorbuf", "RenderError", "render", "render_noalloc", "shader", "Device",
            "default_device", "PIX_RGBA8", "PIX_RGB8", "PIX_U32_RGBA", "PIX_U32_XRGB", "PIX_U32_ARGB",
            "PIX_U32_BGRA", "PIX_RGBA4444", "PIX_RGB565"]
 
@@ -270,6 +270,14 @@ def make_draw(prims, verts, shd, uniform, to_screen, keep: list) -> _ffi.RfDraw:
         d.tex_stride = tex.strides[0] // 4
         keep.append(tex)
     return d
+
+
+def render_noalloc(prims, verts, shd, uniform, to_screen, target, ctx: Context, device: Device | None = None):
+    """render() through rf_render_noalloc: the same call and result, with
+    every device buffer taken from the library's fixed 64 MiB array per GPU
+    instead of device allocations (for callers that cannot allocate). A call
+    whose buffers do not fit raises RenderError(-9)."""
+    _render(prims, verts, shd, uniform, to_screen, target, ctx, device, "rf_render_noalloc")
 
 
 def render(prims, verts, shd, uniform, to_screen, target, ctx: Context, device: Device | None = None):

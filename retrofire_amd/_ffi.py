@@ -80,6 +80,8 @@ SIGNATURES = {
     "rf_tile_height": (C.c_uint32, []),
     "rf_device_set_tile_height": (_st, [_vp, C.c_uint32]),
     "rf_render": (_st, [_vp, C.POINTER(RfDraw), C.POINTER(RfTarget), C.POINTER(RfCtx), C.POINTER(RfStats)]),
+    "rf_render_noalloc": (_st, [_vp, C.POINTER(RfDraw), C.POINTER(RfTarget), C.POINTER(RfCtx),
+                                C.POINTER(RfStats)]),
     "rf_mesh_create": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.c_uint64,
                              C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(_vp)]),
     "rf_mesh_create_edges": (_st, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.c_uint64,

@@ -23,6 +23,7 @@ SOURCES = [
     ("rf_batch", "rf_batch.cpp", []),
     ("rf_serve", "rf_serve.cpp", []),
     ("rf_dsort", "rf_dsort.cpp", []),
+    ("rf_noalloc", "rf_noalloc.cpp", []),
 ]
 HEADERS = ["rf_types.hpp", "rf_math.hpp", "rf_internal.hpp"]
 ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")

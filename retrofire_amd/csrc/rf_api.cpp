@@ -401,6 +401,9 @@ rf_status rf_device_open(int ordinal, rf_device** out) {
         delete d;
         return RF_E_DEVICE;
     }
+    // the device address of rf_render_noalloc's static array on this GPU
+    // (this device is current); on failure that entry point reports RF_E_DEVICE
+    if (noalloc_init(*d) != hipSuccess) d->noalloc = nullptr;
     *out = reinterpret_cast<rf_device*>(d);
     return RF_OK;
 }

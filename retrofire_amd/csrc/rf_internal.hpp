@@ -89,8 +89,13 @@ struct Device {
     uint32_t plain_tile_h = DEFAULT_TILE_H;  // and its tile height
     uint32_t plain_chunk = 8;    // rasterizer span chunk of rf_render and of batches with a forced tile width
     uint32_t plain_extra_cap = 0, plain_runs_cap = 0;
+    // rf_render_noalloc's static device array on this GPU (from
+    // hipGetSymbolAddress at rf_device_open; null if that failed)
+    uint8_t* noalloc = nullptr;
 };
-
+// Looks up the device address of rf_render_noalloc's array on the current
+// device (rf_noalloc.cpp).
+hipError_t noalloc_init(Device& d);
 
 struct Mesh {
     DevBuf pos, attr, tris;
