@@ -240,13 +240,13 @@ rf_status plain_check(const rf_draw* draw, const rf_target* target, const rf_ctx
     return RF_OK;
 }
 
-// Only the bytes a Slice2 view owns are read or written on the host: rows of
-// width elements at the view's stride, (h - 1) * stride + w elements from the
-// first (buf.rs:570-580); the gaps between the rows belong to the parent
-// buffer (an atlas, a pitched SDL surface). A target whose rows are packed
-// (stride = row bytes) keeps that pitch on the device and moves as one
-// block; otherwise the device copy has its own packed pitch and the rows
-// move by a 2D copy.
+// A Slice2 view spans (h - 1) * stride + w elements from its first
+// (buf.rs:570-580): rows of width elements at the view's stride, the gaps
+// between them belonging to the parent buffer (an atlas, a pitched SDL
+// surface). The device copy keeps the view's stride. Uploads read that span
+// as one block (the gaps are the parent's own memory, only read); downloads
+// write only the rows, as one block when they are packed (stride = row
+// bytes) and by a 2D copy otherwise, so no byte outside the view is written.
 PlainLayout plain_layout(const rf_draw* draw, const rf_target* target) {
     PlainLayout L;
     const uint32_t ipp = draw->prim == PRIM_EDGE ? 2u : 3u;  // indices per primitive
@@ -258,11 +258,11 @@ PlainLayout plain_layout(const rf_draw* draw, const rf_target* target) {
     L.tri_b = draw->n_tris * ipp * 4;
     L.tex_b = L.textured ? ((size_t)(draw->tex_h - 1) * draw->tex_stride + draw->tex_w) * 4 : 0;
     L.col_row = (size_t)target->width * bpp;
-    L.col_pitch = target->color_stride == L.col_row ? L.col_row : (L.col_row + 63) & ~size_t(63);
-    L.col_b = (size_t)target->height * L.col_pitch;
+    L.col_pitch = target->color_stride;
+    L.col_b = (size_t)(target->height - 1) * L.col_pitch + L.col_row;
     L.dep_row = (size_t)target->width * 4;
-    L.dep_pitch_f = target->depth_stride == target->width ? target->width : (target->width + 15) & ~15u;
-    L.dep_b = target->depth ? (size_t)target->height * L.dep_pitch_f * 4 : 0;
+    L.dep_pitch_f = target->depth_stride;
+    L.dep_b = target->depth ? ((size_t)(target->height - 1) * L.dep_pitch_f * 4 + L.dep_row) : 0;
     return L;
 }
 
@@ -278,21 +278,23 @@ hipError_t plain_upload_inputs(const rf_draw* draw, const PlainLayout& L, const 
 hipError_t plain_upload_target(const rf_target* target, const rf_ctx* ctx, const PlainLayout& L, const PlainBufs& P,
                                hipStream_t s) {
     hipError_t e = hipSuccess;
-    if (!ctx->clear_color)
-        e = hipMemcpy2DAsync(P.color, L.col_pitch, target->color, target->color_stride, L.col_row, target->height,
-                             hipMemcpyHostToDevice, s);
+    if (!ctx->clear_color) e = hipMemcpyAsync(P.color, target->color, L.col_b, hipMemcpyHostToDevice, s);
     if (!e && target->depth && !ctx->clear_depth)
-        e = hipMemcpy2DAsync(P.depth, L.dep_pitch_f * 4, target->depth, (size_t)target->depth_stride * 4, L.dep_row,
-                             target->height, hipMemcpyHostToDevice, s);
+        e = hipMemcpyAsync(P.depth, target->depth, L.dep_b, hipMemcpyHostToDevice, s);
     return e;
 }
 
 hipError_t plain_download(const rf_target* target, const PlainLayout& L, const PlainBufs& P, hipStream_t s) {
-    hipError_t e = hipMemcpy2DAsync(target->color, target->color_stride, P.color, L.col_pitch, L.col_row,
-                                    target->height, hipMemcpyDeviceToHost, s);
+    // packed rows: one block; a view's rows: a 2D copy (the gaps stay the parent's)
+    hipError_t e = L.col_pitch == L.col_row
+                       ? hipMemcpyAsync(target->color, P.color, L.col_b, hipMemcpyDeviceToHost, s)
+                       : hipMemcpy2DAsync(target->color, L.col_pitch, P.color, L.col_pitch, L.col_row, target->height,
+                                          hipMemcpyDeviceToHost, s);
     if (!e && target->depth)
-        e = hipMemcpy2DAsync(target->depth, (size_t)target->depth_stride * 4, P.depth, L.dep_pitch_f * 4, L.dep_row,
-                             target->height, hipMemcpyDeviceToHost, s);
+        e = L.dep_pitch_f * 4 == L.dep_row
+                ? hipMemcpyAsync(target->depth, P.depth, L.dep_b, hipMemcpyDeviceToHost, s)
+                : hipMemcpy2DAsync(target->depth, L.dep_pitch_f * 4, P.depth, L.dep_pitch_f * 4, L.dep_row,
+                                   target->height, hipMemcpyDeviceToHost, s);
     return e;
 }
 

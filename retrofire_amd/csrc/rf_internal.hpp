@@ -143,8 +143,8 @@ struct PlainLayout {
     int na = 0;          // attribute floats per vertex
     bool textured = false;
     size_t pos_b = 0, attr_b = 0, tri_b = 0, tex_b = 0;
-    size_t col_row = 0, col_pitch = 0, col_b = 0;    // colour: row bytes, device pitch, device bytes
-    size_t dep_row = 0, dep_pitch_f = 0, dep_b = 0;  // depth: row bytes, device pitch (floats), device bytes
+    size_t col_row = 0, col_pitch = 0, col_b = 0;    // colour: row bytes, pitch (the view's), span bytes
+    size_t dep_row = 0, dep_pitch_f = 0, dep_b = 0;  // depth: row bytes, pitch (floats), span bytes
 };
 struct PlainBufs {
     void *pos, *attr, *tris, *tex, *color, *depth, *desc;
