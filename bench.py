@@ -220,11 +220,13 @@ def crates_request(RB, mesh, shd, texr, w, h, yaw):
     return [RB.mesh_draw(mesh, shd, S.crates_mvp(w, h, yaw), M.viewport(0, h, w, 0), texture=texr)]
 
 
-def serve_stream(RB, reqs, seconds, warm, budget=4 << 30):
+def serve_stream(RB, reqs, seconds, warm, budget=4 << 30, alone=None):
     """Open-loop replay of (time, w, h, draws) requests into a fresh server;
     latency = completion - submission, wait = admission - submission. `warm`
     (w, h, draws) requests run first, untimed, so the pinned readback pool
-    and the learned scratch ratios reach their steady state."""
+    and the learned scratch ratios reach their steady state. `alone`, if
+    given, holds each request's latency when served on its own (ms): the
+    slowdown latency / alone is then reported too."""
     srv = RB.Server(budget_bytes=budget, max_batch=32, n_streams=3)
     n = len(reqs)
     for k, (w, h, draws) in enumerate(warm):
@@ -254,11 +256,46 @@ def serve_stream(RB, reqs, seconds, warm, budget=4 << 30):
         return {"requests": n, "completed": len(done), "error": "serving run did not finish"}
     lat = np.array([(r.done_s - r.submit_s) for r in done]) * 1e3
     wait = np.array([(r.start_s - r.submit_s) for r in done]) * 1e3
+    svc = lat - wait
     ok = sum(1 for r in done if r.status == 0)
     pct = lambda a, q: round(float(np.percentile(a, q)), 3)  # noqa: E731
-    return {"requests": n, "ok": ok, "frames_per_s": round(n / (t1 - t0), 1),
-            "p50_ms": pct(lat, 50), "p90_ms": pct(lat, 90), "p99_ms": pct(lat, 99), "max_ms": round(float(lat.max()), 3),
-            "wait_p50_ms": pct(wait, 50), "wait_p99_ms": pct(wait, 99)}
+    # the requests differ in size (five resolutions): latency by resolution
+    # separates the spread of the work from queueing
+    res = np.array([r.width * 65536 + r.height for r in done])
+    by_res = {}
+    for key in sorted(set(res.tolist()), key=lambda k: (k >> 16) * (k & 0xFFFF)):
+        m = res == key
+        by_res[f"{key >> 16}x{key & 0xFFFF}"] = {"n": int(m.sum()), "p50_ms": pct(lat[m], 50), "p99_ms": pct(lat[m], 99)}
+    out = {"requests": n, "ok": ok, "frames_per_s": round(n / (t1 - t0), 1),
+           "p50_ms": pct(lat, 50), "p90_ms": pct(lat, 90), "p99_ms": pct(lat, 99), "max_ms": round(float(lat.max()), 3),
+           "wait_p50_ms": pct(wait, 50), "wait_p99_ms": pct(wait, 99),
+           "service_p50_ms": pct(svc, 50), "service_p99_ms": pct(svc, 99), "by_resolution": by_res,
+           "max_p99_over_p50_within_resolution": round(max(v["p99_ms"] / v["p50_ms"] for v in by_res.values()), 2)}
+    if alone is not None:
+        slow = np.array([lat[k] / alone[r.id] for k, r in enumerate(done)])
+        out.update({"slowdown_p50": pct(slow, 50), "slowdown_p99": pct(slow, 99)})
+    return out
+
+
+def serve_alone(RB, classes, budget=4 << 30, reps=3):
+    """Latency of each request class served on its own (median of `reps`
+    requests, each submitted after the previous one completed): a dict
+    class -> ms, from (class, w, h, draws) tuples."""
+    srv = RB.Server(budget_bytes=budget, max_batch=32, n_streams=3)
+    out = {}
+    for k, (cls, w, h, draws) in enumerate(classes):
+        lat = []
+        for rep in range(reps + 1):
+            srv.submit(k * (reps + 1) + rep, draws, w, h)
+            got = []
+            t_end = time.perf_counter() + 30.0
+            while not got and time.perf_counter() < t_end:
+                got = srv.poll(1, timeout_ms=100)
+            if got and rep > 0:  # the first is a warm-up
+                lat.append((got[0].done_s - got[0].submit_s) * 1e3)
+        out[cls] = float(np.median(lat)) if lat else float("nan")
+    srv.close()
+    return out
 
 
 def serve_run(rf, RB, mesh, texr, shd, a):
@@ -292,11 +329,16 @@ def serve_run(rf, RB, mesh, texr, shd, a):
         return [RB.mesh_draw(dmeshes[mi], gour, S.spheres_mvp(w, h, pose), M.viewport(0, h, w, 0))]
 
     warm = [(w, h, draws_of(w, h, mi, 0.0)) for (w, h) in S.RESOLUTIONS for mi in (7, 15)]
+    # each (resolution, mesh) class alone, at one pose: the reference for the
+    # slowdown of the streamed requests
+    alone_cls = serve_alone(RB, [((w, h, mi), w, h, draws_of(w, h, mi, 0.5)) for (w, h) in S.RESOLUTIONS
+                                 for mi in range(len(meshes))])
     sweep = []
     for rate in (200.0, 500.0, 1000.0, 2000.0):
         spec = S.serve_stream(rate, max(1, int(rate * a.serve_sweep_seconds)), seed=5)
         reqs = [(t_, w, h, draws_of(w, h, mi, pose)) for (t_, w, h, mi, pose) in spec]
-        r = serve_stream(RB, reqs, a.serve_sweep_seconds, warm)
+        r = serve_stream(RB, reqs, a.serve_sweep_seconds, warm,
+                         alone=[alone_cls[(w, h, mi)] for (_, w, h, mi, _) in spec])
         r["offered_rate"] = rate
         sweep.append(r)
     # the plain sequential GPU call on the same kind of requests
@@ -320,6 +362,7 @@ def serve_run(rf, RB, mesh, texr, shd, a):
     out["sweep_16_meshes"] = sweep
     out["sweep_plain_sequential_frames_per_s"] = round(plain_fps, 1)
     out["sweep_seconds_per_rate"] = a.serve_sweep_seconds
+    out["sweep_alone_ms_range"] = [round(min(alone_cls.values()), 3), round(max(alone_cls.values()), 3)]
     return out
 
 
