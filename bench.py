@@ -43,9 +43,12 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--c3-steps", type=int, default=10, help="timed launches of the 64-frame mixed batch")
     p.add_argument("--no-c3", action="store_true")
+    p.add_argument("--c2-steps", type=int, default=10, help="timed launches of 8 icospheres1080 frames")
+    p.add_argument("--no-c2", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--serve-only", action="store_true", help="print only the serving measurement")
     p.add_argument("--calib-copy-mib", type=int, default=0,
                    help="also run one device-to-device copy of this many MiB (scale reference for HBM counters)")
     return p.parse_args()
@@ -71,15 +74,16 @@ def calib_copy(mib):
         raise RuntimeError(f"hipMemcpy failed ({rc})")
 
 
-def measured_traffic(tile_w, chunk):
+def measured_traffic(tile_w, tile_h, chunk):
     """HBM bytes per k_raster launch from the committed counter summary, if it
-    was taken for this tile width and span chunk (profiles/k_raster_traffic.json)."""
+    was taken for this tile shape and span chunk (profiles/k_raster_traffic.json)."""
     try:
         with open(os.path.join(ROOT, "profiles", "k_raster_traffic.json")) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t.get("bytes_per_launch") if (t.get("tile_w"), t.get("chunk")) == (tile_w, chunk) else None
+    key = (t.get("tile_w"), t.get("tile_h"), t.get("chunk"))
+    return t.get("bytes_per_launch") if key == (tile_w, tile_h, chunk) else None
 
 
 def camera_frames(n, w, h, seed=12345):
@@ -113,6 +117,10 @@ def main():
     mesh = RB.Mesh(pos, tris, uv)
     texr = RB.Texture(tex)
     shd = rf.shader.TexRepeatPot(tex)
+    if a.serve_only:
+        # the serving measurement alone (for an API trace of the serving loop)
+        print(json.dumps({"serving": serve_run(rf, RB, mesh, texr, shd, a)}), flush=True)
+        return
     mvps, vp = camera_frames(a.frames, W, H)
     fbs = [RB.DeviceFramebuf(W, H, rf.PIX_RGBA8, True) for _ in range(a.frames)]
     frames = [(fbs[i], rf.Context(clear=True), [RB.mesh_draw(mesh, shd, mvps[i], vp, texture=texr)])
@@ -169,7 +177,7 @@ def main():
         "stage_ms": {n: round(x, 4) for n, x in zip(names, prof)},
         "roofline": {"bound": "hbm", "kernel": "k_raster", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(batch.tile_w, batch.chunk),
+                     "traffic": measured_traffic(batch.tile_w, batch.tile_h, batch.chunk),
                      "traffic_source": "committed rocprofv3 FETCH_SIZE + WRITE_SIZE measurement of this kernel "
                                        "and workload (profiles/k_raster_traffic.json), not measured in this run",
                      "alg_bytes_per_launch": alg_bytes, "launch_ms": round(raster_ms, 4)},
@@ -198,6 +206,8 @@ def main():
 
     # the batched workload at its stated size (64 uneven frames of the full
     # crates and icosphere scenes)
+    if not a.no_c2:
+        out["batched_c2"] = c2_run(rf, RB, a, a.c2_steps)
     if not a.no_c3:
         out["batched_c3"] = c3_run(rf, RB, a, a.c3_steps)
 
@@ -227,6 +237,7 @@ def serve_stream(RB, reqs, seconds, warm, budget=4 << 30, alone=None):
     and the learned scratch ratios reach their steady state. `alone`, if
     given, holds each request's latency when served on its own (ms): the
     slowdown latency / alone is then reported too."""
+    clk = {"open_ns": time.clock_gettime_ns(time.CLOCK_MONOTONIC)}
     srv = RB.Server(budget_bytes=budget, max_batch=32, n_streams=3)
     n = len(reqs)
     for k, (w, h, draws) in enumerate(warm):
@@ -236,6 +247,7 @@ def serve_stream(RB, reqs, seconds, warm, budget=4 << 30, alone=None):
     while got < len(warm) and time.perf_counter() < t_warm:
         got += len(srv.poll(64, timeout_ms=1000))
     t0 = time.perf_counter()
+    clk["stream_ns"] = time.clock_gettime_ns(time.CLOCK_MONOTONIC)
     done = []
     i = 0
     deadline = t0 + seconds + 60.0
@@ -251,6 +263,7 @@ def serve_stream(RB, reqs, seconds, warm, budget=4 << 30, alone=None):
             if nxt > 0.0002:
                 time.sleep(min(nxt, 0.001))
     t1 = time.perf_counter()
+    clk["end_ns"] = time.clock_gettime_ns(time.CLOCK_MONOTONIC)
     srv.close()
     if len(done) < n:
         return {"requests": n, "completed": len(done), "error": "serving run did not finish"}
@@ -270,10 +283,16 @@ def serve_stream(RB, reqs, seconds, warm, budget=4 << 30, alone=None):
            "p50_ms": pct(lat, 50), "p90_ms": pct(lat, 90), "p99_ms": pct(lat, 99), "max_ms": round(float(lat.max()), 3),
            "wait_p50_ms": pct(wait, 50), "wait_p99_ms": pct(wait, 99),
            "service_p50_ms": pct(svc, 50), "service_p99_ms": pct(svc, 99), "by_resolution": by_res,
-           "max_p99_over_p50_within_resolution": round(max(v["p99_ms"] / v["p50_ms"] for v in by_res.values()), 2)}
+           "max_p99_over_p50_within_resolution": round(max(v["p99_ms"] / v["p50_ms"] for v in by_res.values()), 2),
+           "clock_monotonic_ns": clk}
     if alone is not None:
         slow = np.array([lat[k] / alone[r.id] for k, r in enumerate(done)])
-        out.update({"slowdown_p50": pct(slow, 50), "slowdown_p99": pct(slow, 99)})
+        al = np.array(alone, dtype=np.float64)
+        # the requests' own spread: the p99 / p50 latency ratio an ideal
+        # server (every request served alone, at once) would show
+        out.update({"slowdown_p50": pct(slow, 50), "slowdown_p99": pct(slow, 99),
+                    "alone_p50_ms": pct(al, 50), "alone_p99_ms": pct(al, 99),
+                    "alone_p99_over_p50": round(float(np.percentile(al, 99) / np.percentile(al, 50)), 2)})
     return out
 
 
@@ -414,6 +433,40 @@ def c3_run(rf, RB, a, W_steps):
     crates.free()
     icos.free()
     texr.free()
+    return out
+
+
+def c2_run(rf, RB, a, W_steps, n_frames=8):
+    """icospheres1080 (10,000 Gouraud icospheres, 3.2M triangles per frame)
+    batched: n_frames copies of the frame in one HIP-graph launch, tile shape
+    and chunk autotuned on it."""
+    from retrofire_amd import scenes as S
+    sc = S.icospheres1080()
+    mesh = RB.Mesh(sc.pos, sc.tris, sc.attr)
+    fbs = [RB.DeviceFramebuf(sc.width, sc.height, 0, True) for _ in range(n_frames)]
+    b = RB.Batch([(fb, rf.Context(clear=True), [RB.mesh_draw(mesh, rf.shader.Gouraud(), sc.mvp, sc.viewport)]) for fb in fbs])
+    b.run(sync=True)
+    stats = b.wait()
+    t0 = time.perf_counter()
+    for _ in range(W_steps):
+        b.run(sync=False)
+    b.wait()
+    el = time.perf_counter() - t0
+    prof = b.profile(iters=3)
+    alg = n_frames * sc.width * sc.height * 8
+    out = {"frames_per_step": n_frames, "steps": W_steps, "ms_per_step": round(1e3 * el / W_steps, 3),
+           "frames_per_s": round(n_frames * W_steps / el, 1),
+           "mfrag_per_s": round(sum(s.frags_i for s in stats) * W_steps / el / 1e6, 1),
+           "tile_w": b.tile_w, "tile_h": b.tile_h, "chunk": b.chunk,
+           "autotune_ms": [list(t) for t in b.tune_results()],
+           "stage_ms": {n: round(x, 4) for n, x in zip(["vertex", "setup", "scan", "bin", "binsort", "raster"], prof)},
+           "raster_roofline": {"alg_bytes_per_launch": alg, "launch_ms": round(prof[5], 4),
+                               "achieved_GBs": round(alg / (prof[5] * 1e-3) / 1e9, 1),
+                               "frac": round(alg / (prof[5] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    b.free()
+    for fb in fbs:
+        fb.free()
+    mesh.free()
     return out
 
 

@@ -1,9 +1,11 @@
 // Serving: a request loop over resident meshes and textures.
 //
 // Requests (one frame each: a resolution, a context and an ordered list of
-// draws over resident meshes) are admitted in arrival order while their
-// device memory fits the budget; whatever is admitted when a stream is free
-// goes out as one pipeline launch (continuous batching). Up to `n_streams`
+// draws over resident meshes) are admitted oldest first while their device
+// memory fits the budget; what is admitted when a stream is free goes out as
+// one pipeline launch (continuous batching). A launch groups requests of
+// similar cost only: every frame of a launch completes when the launch does,
+// so a small frame batched with large ones would wait for them. Up to `n_streams`
 // launches are on the device at once, so frames that arrive while others are
 // rendering overlap them on the GPU. Each stream has a waiter thread that
 // blocks on an event recorded behind its launch; results carry
@@ -51,6 +53,19 @@ namespace rf {
 namespace {
 
 constexpr uint32_t SERVE_TILE_W = RF_SERVE_TILE_W;  // rasterizer tile width of served launches
+// Estimated GPU cost of a request in pixel units (about 10 ps each): a
+// triangle's vertex, setup, binning and edge work costs about as much as 28
+// pixels' fill and write-back (crates1080 and icospheres1080 batches), and a
+// launch has a fixed part (its kernel sequence) worth about 0.1 ms.
+constexpr double COST_TRI = 28.0;
+constexpr double COST_LAUNCH = 1.0e7;
+// A launch's estimated cost stays within this factor of its cheapest
+// request's cost alone (the slowdown batching may cause that request).
+#ifndef RF_SERVE_SLOWDOWN
+#define RF_SERVE_SLOWDOWN 3.0
+#endif
+constexpr double SERVE_SLOWDOWN = RF_SERVE_SLOWDOWN;
+constexpr size_t SERVE_SCAN = 256;  // pending requests looked at per launch
 constexpr uint64_t ALIGN = 256;
 inline uint64_t align_up(uint64_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
 
@@ -517,33 +532,49 @@ static void worker_main(Serve* sp) {
                 }
             }
         }
-        // 3. admit pending requests onto free lanes, in arrival order, while
+        // 3. admit pending requests onto free lanes, the oldest first, while
         //    the launch's block fits the largest free range of the arena (not
-        //    while a rerun waits for memory: it goes first)
+        //    while a rerun waits for memory: it goes first). Behind the oldest
+        //    one, a request joins only if the launch's estimated cost stays
+        //    within SERVE_SLOWDOWN times its cheapest member's cost alone;
+        //    the others stay pending, in order, for the next launch.
         bool rerun_waits = false;
         for (Lane* l : s.lanes) rerun_waits = rerun_waits || l->wait_mem;
         for (Lane* l : s.lanes) {
             if (l->busy || s.pending.empty() || rerun_waits) continue;
             const uint64_t room = s.arena.largest();
             Plan p;
-            while (!s.pending.empty() && l->reqs.size() < s.max_batch) {
-                Req& q = s.pending.front();
-                Plan p2 = p;
-                p2.add(q);
+            double cost = COST_LAUNCH, cheapest = 0.0;
+            std::vector<size_t> take;
+            size_t i = 0;
+            while (i < s.pending.size() && i < SERVE_SCAN && take.size() < s.max_batch) {
+                Req& q = s.pending[i];
                 Plan alone;
                 alone.add(q);
                 if (block_bytes(alone, est_extra(s, alone), est_runs(s, alone)) > s.arena.size) {
                     reject(s, q, RF_E_BUDGET);  // can never fit
-                    s.pending.pop_front();
+                    s.pending.erase(s.pending.begin() + (ptrdiff_t)i);
                     progressed = true;
                     continue;
                 }
+                const double c = (double)q.r.width * q.r.height + COST_TRI * (double)q.n_tris;
+                const double cheap = take.empty() ? c : std::min(cheapest, c);
+                if (!take.empty() && cost + c > SERVE_SLOWDOWN * (COST_LAUNCH + cheap)) {
+                    ++i;  // another cost class: a later launch
+                    continue;
+                }
+                Plan p2 = p;
+                p2.add(q);
                 if (p2.n_tris >= MAX_LAUNCH_PRIMS) break;
                 if (block_bytes(p2, est_extra(s, p2), est_runs(s, p2)) > room) break;  // wait for memory
                 p = p2;
-                l->reqs.push_back(std::move(q));
-                s.pending.pop_front();
+                cost += c;
+                cheapest = cheap;
+                take.push_back(i);
+                ++i;
             }
+            for (size_t k : take) l->reqs.push_back(std::move(s.pending[k]));
+            for (auto it = take.rbegin(); it != take.rend(); ++it) s.pending.erase(s.pending.begin() + (ptrdiff_t)*it);
             if (l->reqs.empty()) continue;
             l->extra_cap = est_extra(s, p);
             l->runs_cap = est_runs(s, p);
