@@ -17,7 +17,12 @@
 // targets of its frames -- and gives it back once its results are read back.
 // So the budget bounds the loop's device memory, and the steady-state loop
 // makes no device allocation (no hipMalloc or hipFree, which would
-// synchronise the device under the other streams' work). The scratch of a
+// synchronise the device under the other streams' work). Read-back buffers
+// come from one pinned host block allocated at open as well (a quarter of the
+// budget, 64 MiB to 1 GiB), reserved when a request is admitted and returned
+// when its result is polled; only a frame larger than that whole block gets
+// a pinned buffer of its own. The descriptor staging of each stream is sized
+// at open for max_batch frames of up to 8 draws each. The scratch of a
 // launch is sized from its counts and from clip-fan and bin-run ratios
 // learned from earlier launches; a launch that needs more is rerun with a
 // larger block.
@@ -129,6 +134,7 @@ struct Done {
     uint8_t* pixels = nullptr;  // pinned, width * height * bpp
     size_t pix_bytes = 0;
 };
+constexpr uint32_t STAGE_DRAWS = 8;  // draws per frame the staging is sized for at open
 
 // Counts of a launch: what its block must hold.
 struct Plan {
@@ -192,8 +198,9 @@ struct Serve {
     std::deque<Done> done;
     bool stop = false;
     bool stop_waiters = false;
-    std::mutex pm;  // guards the pinned pool; never held across a HIP call
-    std::map<size_t, std::vector<uint8_t*>> pin_pool;
+    std::mutex pm;  // guards `pin`; never held across a HIP call (taken after `m`)
+    void* pin_mem = nullptr;
+    Arena pin;      // read-back buffers in the pinned block
 };
 
 static void waiter_main(Serve* sp, Lane* l) {
@@ -215,24 +222,27 @@ static void waiter_main(Serve* sp, Lane* l) {
     }
 }
 
-static uint8_t* pin_get(Serve& s, size_t bytes) {
-    {
-        std::lock_guard<std::mutex> g(s.pm);
-        auto& v = s.pin_pool[bytes];
-        if (!v.empty()) {
-            uint8_t* p = v.back();
-            v.pop_back();
-            return p;
-        }
-    }
-    uint8_t* p = nullptr;
-    if (hipHostMalloc((void**)&p, bytes, 0)) return nullptr;
-    return p;
+// Read-back buffer of `bytes` from the pinned block (nullptr if it does not
+// fit now). Called with `m` held.
+static uint8_t* pin_take(Serve& s, size_t bytes) {
+    std::lock_guard<std::mutex> g(s.pm);
+    uint64_t off = 0;
+    if (!s.pin.alloc(align_up(bytes), off)) return nullptr;
+    return s.pin.base + off;
 }
+static bool pin_in_block(const Serve& s, const uint8_t* p) {
+    return p >= s.pin.base && p < s.pin.base + s.pin.size;
+}
+// Returns a read-back buffer: to the pinned block, or (a frame larger than
+// the block) to the host.
 static void pin_put(Serve& s, uint8_t* p, size_t bytes) {
     if (!p) return;
-    std::lock_guard<std::mutex> g(s.pm);
-    s.pin_pool[bytes].push_back(p);
+    if (pin_in_block(s, p)) {
+        std::lock_guard<std::mutex> g(s.pm);
+        s.pin.release((uint64_t)(p - s.pin.base), align_up(bytes));
+    } else {
+        (void)hipHostFree(p);
+    }
 }
 
 static uint32_t est_extra(const Serve& s, const Plan& p) {
@@ -433,8 +443,8 @@ static rf_status launch_lane(Serve& s, Lane& l) {
         }
     }
     B.any_cull = any_cull;
-    // pinned staging of the descriptors; it grows only when a launch needs
-    // more than any before (host memory, never the device's)
+    // pinned staging of the descriptors, sized at open; it grows only for a
+    // launch of more draws than that (host memory, never the device's)
     const size_t total = db + fb;
     if (total > l.h_desc_cap) {
         if (l.h_desc) (void)hipHostFree(l.h_desc);
@@ -567,6 +577,13 @@ static void worker_main(Serve* sp) {
                 p2.add(q);
                 if (p2.n_tris >= MAX_LAUNCH_PRIMS) break;
                 if (block_bytes(p2, est_extra(s, p2), est_runs(s, p2)) > room) break;  // wait for memory
+                // its read-back buffer: reserved now, or (larger than the
+                // whole pinned block) allocated at launch
+                const size_t pb = (size_t)q.r.width * q.r.height * bytes_per_pixel(q.r.fmt);
+                uint8_t* pin = nullptr;
+                if (align_up(pb) <= s.pin.size && !(pin = pin_take(s, pb))) break;  // wait for polls
+                l->pix.push_back(pin);
+                l->pix_bytes.push_back(pb);
                 p = p2;
                 cost += c;
                 cheapest = cheap;
@@ -581,6 +598,9 @@ static void worker_main(Serve* sp) {
             if (!take_block(s, *l)) {  // cannot happen: the block fits `room`
                 for (auto it = l->reqs.rbegin(); it != l->reqs.rend(); ++it) s.pending.push_front(std::move(*it));
                 l->reqs.clear();
+                for (size_t k = 0; k < l->pix.size(); ++k) pin_put(s, l->pix[k], l->pix_bytes[k]);
+                l->pix.clear();
+                l->pix_bytes.clear();
                 continue;
             }
             l->busy = true;
@@ -595,13 +615,11 @@ static void worker_main(Serve* sp) {
             std::vector<std::pair<Lane*, rf_status>> failed;
             for (Lane* l : launch) {
                 rf_status st = RF_OK;
-                if (l->pix.empty())
-                    for (Req& q : l->reqs) {
-                        const size_t pb = (size_t)q.r.width * q.r.height * bytes_per_pixel(q.r.fmt);
-                        uint8_t* pin = pin_get(s, pb);
-                        l->pix.push_back(pin);
-                        l->pix_bytes.push_back(pb);
-                        if (!pin) st = RF_E_DEVICE;
+                // frames larger than the pinned block: a buffer of their own
+                for (size_t k = 0; k < l->pix.size(); ++k)
+                    if (!l->pix[k] && hipHostMalloc((void**)&l->pix[k], l->pix_bytes[k], 0)) {
+                        l->pix[k] = nullptr;
+                        st = RF_E_DEVICE;
                     }
                 if (st == RF_OK) st = launch_lane(s, *l);
                 if (st != RF_OK) {
@@ -655,15 +673,27 @@ rf_status rf_serve_open(rf_device* dev, uint64_t budget_bytes, uint32_t max_batc
         return RF_E_DEVICE;
     }
     s->arena.init((uint8_t*)s->arena_mem, budget_bytes);
+    const uint64_t pin_bytes = std::min<uint64_t>(1ull << 30, std::max<uint64_t>(64ull << 20, budget_bytes / 4));
+    if (hipHostMalloc(&s->pin_mem, pin_bytes, 0)) {
+        s->pin_mem = nullptr;
+        rf_serve_close(reinterpret_cast<rf_serve*>(s));
+        return RF_E_DEVICE;
+    }
+    s->pin.init((uint8_t*)s->pin_mem, pin_bytes);
+    const size_t stage = 2 * (align_up((size_t)max_batch * STAGE_DRAWS * sizeof(DrawDesc)) +
+                              align_up((size_t)max_batch * sizeof(FrameDesc)));
     for (uint32_t i = 0; i < n_streams; ++i) {
         Lane* l = new Lane();
         s->lanes.push_back(l);
         if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) ||
             hipEventCreateWithFlags(&l->ev, hipEventBlockingSync | hipEventDisableTiming) ||
-            hipHostMalloc((void**)&l->h_misc, 64, 0)) {
+            hipHostMalloc((void**)&l->h_misc, 64, 0) || hipHostMalloc((void**)&l->h_desc, stage, 0) ||
+            hipHostMalloc((void**)&l->h_stats, (size_t)max_batch * 2 * sizeof(FrameStats), 0)) {
             rf_serve_close(reinterpret_cast<rf_serve*>(s));
             return RF_E_DEVICE;
         }
+        l->h_desc_cap = stage;
+        l->stats_cap = max_batch * 2;
     }
     for (Lane* l : s->lanes) l->waiter = std::thread(waiter_main, s, l);
     s->worker = std::thread(worker_main, s);
@@ -724,6 +754,8 @@ rf_status rf_serve_poll(rf_serve* sh, rf_result* out, void** color, uint32_t max
         ++n;
     }
     *n_out = n;
+    lk.unlock();
+    if (n) s.cv_work.notify_all();  // admissions may wait for read-back space
     return RF_OK;
 }
 
@@ -757,9 +789,8 @@ rf_status rf_serve_close(rf_serve* sh) {
     }
     if (s->arena_mem) (void)hipFree(s->arena_mem);
     for (auto& d2 : s->done)
-        if (d2.pixels) (void)hipHostFree(d2.pixels);
-    for (auto& kv : s->pin_pool)
-        for (uint8_t* p : kv.second) (void)hipHostFree(p);
+        if (d2.pixels && !pin_in_block(*s, d2.pixels)) (void)hipHostFree(d2.pixels);
+    if (s->pin_mem) (void)hipHostFree(s->pin_mem);
     delete s;
     return RF_OK;
 }
