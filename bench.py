@@ -362,13 +362,13 @@ def serve_run(rf, RB, mesh, texr, shd, a):
         sweep.append(r)
     # the plain sequential GPU call on the same kind of requests
     spec = S.serve_stream(1000.0, 200, seed=5)
-    col = np.zeros((2048, 2048, 4), np.uint8)
-    dep = np.zeros((2048, 2048), np.float32)
+    # one packed colour and depth array per resolution, as a caller would hold
+    tgt = {(w, h): (np.zeros((h, w, 4), np.uint8), np.zeros((h, w), np.float32)) for (w, h) in S.RESOLUTIONS}
     crates_shd = rf.shader.TexRepeatPot(tex_px)
     t0 = time.perf_counter()
     for (_, w, h, mi, pose) in spec:
         kind, p, at, tr = meshes[mi]
-        c, d = col[:h, :w], dep[:h, :w]
+        c, d = tgt[(w, h)]
         if kind == "crates":
             rf.render(tr, (p, at), crates_shd, S.crates_mvp(w, h, pose), M.viewport(0, h, w, 0), rf.Framebuf(c, d),
                       rf.Context(clear=True))
