@@ -175,6 +175,47 @@ def test_serving_budget_rejects_oversized():
     m.free()
 
 
+def test_serving_readback_space_waits_for_polls(world):
+    """Results wait in the server's pinned read-back block until they are
+    polled. With a 256 MiB budget the block is 64 MiB: four 2048x2048 RGBA
+    frames. Twelve such requests submitted before any poll: the server
+    launches what fits, the rest start only after polls free space, and every
+    frame still equals the plain path."""
+    import time
+    crates = RB.Mesh(world["cpos"], world["ctris"], world["cuv"])
+    tex = RB.Texture(world["tex"])
+    shd = rf.shader.TexRepeatPot(world["tex"])
+    w = h = 2048
+    vp = M.viewport(0, h, w, 0)
+    mvps = [M.then(S.first_person(0.0, 1.7, 0.0, k / 12.0), _persp(w, h)) for k in range(12)]
+    srv = RB.Server(budget_bytes=256 << 20, max_batch=4, n_streams=2)
+    got = {}
+    try:
+        for k in range(12):
+            srv.submit(k, [RB.mesh_draw(crates, shd, mvps[k], vp, texture=tex)], w, h)
+        time.sleep(0.5)  # the server fills its read-back block and waits
+        t_poll = time.monotonic()
+        t_end = t_poll + 60.0
+        while len(got) < 12:
+            assert time.monotonic() < t_end, f"results missing: {sorted(set(range(12)) - set(got))}"
+            bufs = [np.empty(w * h * 4, np.uint8) for _ in range(2)]
+            for r, b in zip(srv.poll(2, timeout_ms=500, buffers=bufs), bufs):
+                assert r.status == 0, f"request {r.id} status {r.status}"
+                got[r.id] = (r, b.reshape(h, w, 4).copy())
+    finally:
+        srv.close()
+    late = sum(1 for r, _ in got.values() if r.start_s >= t_poll)
+    assert late >= 8, f"only {late} of 12 requests started after the first poll"
+    for k in range(12):
+        col = np.zeros((h, w, 4), np.uint8)
+        dep = np.zeros((h, w), np.float32)
+        rf.render(world["ctris"], (world["cpos"], world["cuv"]), shd, mvps[k], vp, rf.Framebuf(col, dep),
+                  rf.Context(clear=True))
+        assert np.array_equal(got[k][1], col), f"request {k}: served != plain"
+    crates.free()
+    tex.free()
+
+
 def test_object_culling_matches_caller_side_culling():
     """Per-object draws with `cull` set: the device skips each object whose
     bounding box is Hidden, exactly as the crates demo skips it before calling
