@@ -1004,6 +1004,26 @@ __device__ __forceinline__ void tri_setup(Slot& S, float (&va)[6], float (&vb)[6
     S.yhi = hi;
 }
 
+// 16-byte framebuffer stores. The rasterizer writes every target pixel once
+// and never reads it back in the launch, so with RF_NT_STORE the stores are
+// streaming (non-temporal): a gigabyte of colour and depth per launch then
+// does not displace from L2 the triangles, runs and texels that other tiles
+// still read.
+#ifndef RF_NT_STORE
+#define RF_NT_STORE 1
+#endif
+typedef uint32_t rf_u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st16(void* p, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+#if RF_NT_STORE
+    __builtin_nontemporal_store(rf_u32x4{a, b, c, d}, (rf_u32x4*)p);
+#else
+    *(uint4*)p = make_uint4(a, b, c, d);
+#endif
+}
+__device__ __forceinline__ void st16f(float* p, float a, float b, float c, float d) {
+    st16(p, __float_as_uint(a), __float_as_uint(b), __float_as_uint(c), __float_as_uint(d));
+}
+
 __device__ __forceinline__ void store_pixel(const FrameDesc& F, uint32_t x, uint32_t y, uint32_t v) {
     uint8_t* p = F.color + (uint64_t)y * F.color_stride;
     if (F.fmt == PIX_RGB8) {
@@ -1060,9 +1080,8 @@ __device__ __forceinline__ void clear_tile(const FrameDesc& F, uint32_t tile, ui
         for (uint32_t q = lane; q < nrows * QR; q += WAVE) {
             const uint32_t r = q / QR, c = (q - r * QR) * 4;
             if (F.clear_color)
-                *(uint4*)(F.color + (uint64_t)(y0 + r) * F.color_stride + 4ull * (x0 + c)) = make_uint4(cp, cp, cp, cp);
-            if (has_depth && F.clear_depth)
-                *(float4*)(F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 + c) = make_float4(cz, cz, cz, cz);
+                st16(F.color + (uint64_t)(y0 + r) * F.color_stride + 4ull * (x0 + c), cp, cp, cp, cp);
+            if (has_depth && F.clear_depth) st16f(F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 + c, cz, cz, cz, cz);
         }
     } else {
         for (uint32_t i = lane; i < ncols * nrows; i += WAVE) {
@@ -1379,11 +1398,10 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                             for (uint32_t q = lane; q < TILE_H * qpr; q += WAVE) {
                                 const uint32_t r = q / qpr, c = (q - r * qpr) * 4;
                                 if (E.clear_color)
-                                    *(uint4*)(E.color + (uint64_t)(y0 + r) * E.color_stride + 4ull * (x0 + c)) =
-                                        make_uint4(cp, cp, cp, cp);
+                                    st16(E.color + (uint64_t)(y0 + r) * E.color_stride + 4ull * (x0 + c), cp, cp, cp,
+                                         cp);
                                 if (has_depth && E.clear_depth)
-                                    *(float4*)(E.depth + (uint64_t)(y0 + r) * E.depth_stride + x0 + c) =
-                                        make_float4(cz, cz, cz, cz);
+                                    st16f(E.depth + (uint64_t)(y0 + r) * E.depth_stride + x0 + c, cz, cz, cz, cz);
                             }
                         }
                         continue;
@@ -1996,11 +2014,10 @@ __global__ __launch_bounds__(64, 2) void k_raster(BatchDesc B) {
                         px[k] = F.clear_color ? F.clear_pixel : oldk[k];
                     }
                 }
-                *(uint4*)crow = make_uint4(px[0], px[1], px[2], px[3]);
+                st16(crow, px[0], px[1], px[2], px[3]);
                 if (has_depth)
-                    *(float4*)(F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 + c) =
-                        make_float4(s_dep[c * LD + r], s_dep[(c + 1) * LD + r], s_dep[(c + 2) * LD + r],
-                                    s_dep[(c + 3) * LD + r]);
+                    st16f(F.depth + (uint64_t)(y0 + r) * F.depth_stride + x0 + c, s_dep[c * LD + r],
+                          s_dep[(c + 1) * LD + r], s_dep[(c + 2) * LD + r], s_dep[(c + 3) * LD + r]);
             }
         } else {
             for (uint32_t i = lane; i < npx; i += WAVE) {
