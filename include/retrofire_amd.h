@@ -300,10 +300,14 @@ rf_status rf_batch_profile(rf_batch* b, int iters, float* ms);
 
 /* ---- serving ------------------------------------------------------------
  * A request loop that keeps meshes and textures resident, admits frames by a
- * device-memory budget and batches whatever is admitted while earlier work is
- * still on the GPU (continuous batching over `n_streams` streams). No
- * counterpart in the reference; every request is a frame of render() calls
- * cleared first, as the frontends' loops do (front/src/lib.rs:103-120). */
+ * device-memory budget and batches what is admitted while earlier work is
+ * still on the GPU (continuous batching over `n_streams` streams; a launch
+ * groups requests of similar cost). No counterpart in the reference; every
+ * request is a frame of render() calls cleared first, as the frontends' loops
+ * do (front/src/lib.rs:103-120). rf_serve_open allocates all the loop's
+ * memory: the device budget and a pinned read-back block of budget / 4 (64
+ * MiB to 1 GiB). A result's pixels occupy that block until rf_serve_poll
+ * returns them, so results left unpolled hold back new launches. */
 typedef struct rf_request {
     uint64_t id;
     const rf_mesh_draw* draws;   /* copied at submit */
