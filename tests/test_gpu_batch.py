@@ -216,6 +216,42 @@ def test_serving_readback_space_waits_for_polls(world):
     tex.free()
 
 
+def test_serving_frame_larger_than_readback_block(world):
+    """A frame larger than the whole pinned read-back block (64 MiB with a
+    256 MiB budget) gets a pinned buffer of its own and equals the plain
+    path; a small frame served beside it still uses the block."""
+    crates = RB.Mesh(world["cpos"], world["ctris"], world["cuv"])
+    tex = RB.Texture(world["tex"])
+    shd = rf.shader.TexRepeatPot(world["tex"])
+    sizes = [(4096, 4112), (256, 256)]  # 4096 * 4112 * 4 B = 64.25 MiB
+    srv = RB.Server(budget_bytes=256 << 20, max_batch=2, n_streams=1)
+    got = {}
+    try:
+        for k, (w, h) in enumerate(sizes):
+            mvp = M.then(S.first_person(0.0, 1.7, 0.0, 0.1 * k), _persp(w, h))
+            srv.submit(k, [RB.mesh_draw(crates, shd, mvp, M.viewport(0, h, w, 0), texture=tex)], w, h)
+        import time
+        t_end = time.monotonic() + 60.0
+        while len(got) < 2:
+            assert time.monotonic() < t_end, "results missing"
+            bufs = [np.empty(4096 * 4112 * 4, np.uint8)]
+            for r, b in zip(srv.poll(1, timeout_ms=500, buffers=bufs), bufs):
+                assert r.status == 0, f"request {r.id} status {r.status}"
+                w, h = sizes[r.id]
+                got[r.id] = b[: w * h * 4].reshape(h, w, 4).copy()
+    finally:
+        srv.close()
+    for k, (w, h) in enumerate(sizes):
+        mvp = M.then(S.first_person(0.0, 1.7, 0.0, 0.1 * k), _persp(w, h))
+        col = np.zeros((h, w, 4), np.uint8)
+        dep = np.zeros((h, w), np.float32)
+        rf.render(world["ctris"], (world["cpos"], world["cuv"]), shd, mvp, M.viewport(0, h, w, 0),
+                  rf.Framebuf(col, dep), rf.Context(clear=True))
+        assert np.array_equal(got[k], col), f"request {k} ({w}x{h}): served != plain"
+    crates.free()
+    tex.free()
+
+
 def test_object_culling_matches_caller_side_culling():
     """Per-object draws with `cull` set: the device skips each object whose
     bounding box is Hidden, exactly as the crates demo skips it before calling
