@@ -74,8 +74,9 @@ constexpr size_t SERVE_SCAN = 256;  // pending requests looked at per launch
 constexpr uint64_t ALIGN = 256;
 inline uint64_t align_up(uint64_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
 
-// First-fit allocator over the arena's byte range; free ranges coalesce.
-// Used by the worker thread only.
+// First-fit allocator over a byte range; free ranges coalesce. Not locked:
+// the device arena is used by the worker thread only, the pinned read-back
+// block under Serve::pm.
 struct Arena {
     uint8_t* base = nullptr;
     uint64_t size = 0, used = 0;
