@@ -92,6 +92,9 @@ struct Device {
     // rf_render_noalloc's static device array on this GPU (from
     // hipGetSymbolAddress at rf_device_open; null if that failed)
     uint8_t* noalloc = nullptr;
+    // pinned host staging of rf_render's read-back into a view (grow-only)
+    uint8_t* stage = nullptr;
+    size_t stage_cap = 0;
 };
 // Looks up the device address of rf_render_noalloc's array on the current
 // device (rf_noalloc.cpp).
@@ -156,6 +159,11 @@ hipError_t plain_upload_inputs(const rf_draw* draw, const PlainLayout& L, const 
 hipError_t plain_upload_target(const rf_target* target, const rf_ctx* ctx, const PlainLayout& L, const PlainBufs& P,
                                hipStream_t s);
 hipError_t plain_download(const rf_target* target, const PlainLayout& L, const PlainBufs& P, hipStream_t s);
+// rf_render's read-back, synchronous: packed rows straight into the target;
+// a view's rows by one DMA transfer into d.stage (pinned, up to 256 MiB), then
+// on the host into the view (larger views, or no pinned memory: a 2D copy).
+hipError_t plain_download_sync(Device& d, const rf_target* target, const PlainLayout& L, const PlainBufs& P,
+                               hipStream_t s);
 void plain_descs(const rf_draw* draw, const rf_target* target, const rf_ctx* ctx, const PlainLayout& L,
                  const PlainBufs& P, uint32_t tile_w, uint32_t tile_h, uint32_t chunk, DrawDesc& dd, FrameDesc& fd,
                  BatchDesc& B);
