@@ -40,9 +40,10 @@ typedef int32_t rf_status;
 #define RF_E_ARG (-3)         /* invalid argument (e.g. non-power-of-two texture for
                                  SamplerRepeatPot, tex.rs:228-233 panics)            */
 #define RF_E_DEVICE (-4)      /* HIP runtime error or out of device memory         */
-#define RF_E_UNSUPPORTED (-5) /* not a built-in shader kind / target layout, or
-                                 depth_sort on the batched or serving path: the
-                                 caller keeps the CPU path                          */
+#define RF_E_UNSUPPORTED (-5) /* not a built-in shader kind / target layout, a
+                                 launch of 2^29 primitives or more, or depth_sort
+                                 in rf_render_noalloc: the caller takes another
+                                 path                                               */
 #define RF_E_VIEWPORT (-6)    /* to_screen puts a vertex more than 2^22 rows above
                                  the target (rows above row 0 fold onto it, as
                                  `self.y as usize` does, raster.rs:106, up to that) */
@@ -84,8 +85,9 @@ typedef enum rf_pixfmt {
  * 0 = None, 1 = FrontToBack, 2 = BackToFront; rf_render sorts the clipped
  * triangles by Render::depth under f32::total_cmp (render.rs:209-219), equal
  * depths in submission order (the reference's sort_unstable_by leaves their
- * order to the Rust toolchain); the batched and serving paths return
- * RF_E_UNSUPPORTED for it. clear_color / clear_depth fuse the frontend's
+ * order to the Rust toolchain); batched and served frames sort each of their
+ * triangle draws the same way (a served depth-sorted request is launched on
+ * its own). clear_color / clear_depth fuse the frontend's
  * Frame::clear (front/src/lib.rs:103-120) into the call: the colour buffer is
  * first filled with color_clear, the depth buffer with 1 / depth_clear. */
 typedef struct rf_ctx {

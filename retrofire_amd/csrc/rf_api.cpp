@@ -177,10 +177,10 @@ uint32_t raster_grid(const Device& d, const BatchDesc& B) {
 // targets) before a rerun.
 rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s,
                     const std::function<hipError_t(const BatchDesc&)>& pre, std::vector<FrameStats>& stats_out,
-                    uint32_t* err_out, const std::function<hipError_t()>& restore) {
+                    uint32_t* err_out, const std::function<hipError_t()>& restore, uint64_t* counter) {
     if (B.n_tris >= MAX_LAUNCH_PRIMS) return RF_E_UNSUPPORTED;
     for (int attempt = 0; attempt < 4; ++attempt) {
-        if (sc.prepare(B, &d.bytes) != hipSuccess) return RF_E_DEVICE;
+        if (sc.prepare(B, counter) != hipSuccess) return RF_E_DEVICE;
         if (pre && pre(B) != hipSuccess) return RF_E_DEVICE;
         if (launch_pipeline(B, s, raster_grid(d, B)) != hipSuccess) return RF_E_DEVICE;
         uint32_t misc[3];
@@ -573,7 +573,7 @@ rf_status rf_render(rf_device* dev, const rf_draw* draw, rf_target* target, cons
 
     std::vector<FrameStats> fs;
     uint32_t err = 0;
-    st = run_sized(d, d.plain, B, s, pre, fs, &err, upload_target);
+    st = run_sized(d, d.plain, B, s, pre, fs, &err, upload_target, &d.bytes);
     if (st) return st;
     d.plain_extra_cap = std::max(d.plain_extra_cap, B.extra_cap);
     d.plain_runs_cap = std::max(d.plain_runs_cap, B.runs_cap);

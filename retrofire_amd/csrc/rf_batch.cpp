@@ -172,7 +172,7 @@ static rf_status size_batch(Batch& b) {
     std::vector<FrameStats> fs;
     uint32_t err = 0;
     rf_status st = run_sized(*b.dev, b.sc, b.B, b.stream, [&b](const BatchDesc& M) { return sort_pass(b, M); },
-                             fs, &err, nullptr);
+                             fs, &err, nullptr, &b.dev->bytes);
     if (st) return st;
     if ((st = map_err(err))) return st;
     return RF_OK;

@@ -134,9 +134,12 @@ void set_grid(FrameDesc& f, uint32_t tile_w, uint32_t tile_h, uint32_t& tile_bas
 uint32_t raster_grid(const Device& d, const BatchDesc& B);
 // Resident one-wave rasterizer workgroups per CU of an instance (rf_kernels.hip).
 uint32_t raster_wg_per_cu(uint32_t tile_w, uint32_t chunk, uint32_t tile_h);
+// Runs B's pipeline on `s` synchronously, growing B's clip-fan and bin-run
+// capacities (and the scratch, counted in *counter) until they suffice;
+// `pre` runs before each attempt's pipeline, `restore` before each retry.
 rf_status run_sized(Device& d, Scratch& sc, BatchDesc& B, hipStream_t s,
                     const std::function<hipError_t(const BatchDesc&)>& pre, std::vector<FrameStats>& stats_out,
-                    uint32_t* err_out, const std::function<hipError_t()>& restore);
+                    uint32_t* err_out, const std::function<hipError_t()>& restore, uint64_t* counter);
 
 // The plain path (one render() call with host buffers): argument checks,
 // the device layout of its inputs and target, the host<->device copies, the

@@ -25,7 +25,8 @@
 // at open for max_batch frames of up to 8 draws each. The scratch of a
 // launch is sized from its counts and from clip-fan and bin-run ratios
 // learned from earlier launches; a launch that needs more is rerun with a
-// larger block.
+// larger block. A depth-sorted request (render.rs:180-183) is launched alone,
+// through its stream's sort pass and grow-only scratch (launch_lane_sorted).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -128,6 +129,7 @@ struct Req {
     std::vector<rf_mesh_draw> draws;
     double submit_s = 0;
     uint64_t n_tris = 0, n_verts = 0;
+    bool sorted = false;  // depth_sort with a triangle draw: launched alone, through the lane's sort pass
 };
 
 struct Done {
@@ -140,7 +142,9 @@ constexpr uint32_t STAGE_DRAWS = 8;  // draws per frame the staging is sized for
 // Counts of a launch: what its block must hold.
 struct Plan {
     uint64_t n_verts = 0, n_tris = 0, n_draws = 0, n_frames = 0, n_tiles = 0, targets = 0;
+    bool sorted = false;  // a depth-sorted launch: its scratch is the lane's own
     void add(const Req& q) {
+        sorted = sorted || q.sorted;
         n_verts += q.n_verts;
         n_tris += q.n_tris;
         n_draws += q.draws.size();
@@ -168,6 +172,10 @@ struct Lane {
     uint64_t blk_off = 0, blk_bytes = 0;  // arena block (blk_bytes 0: none)
     uint32_t extra_cap = 0, runs_cap = 0;
     BatchDesc B{};
+    // depth-sorted launches (render.rs:180-183): the sort pass and the
+    // pipeline scratch, grow-only device buffers of this lane
+    SortPass sp;
+    Scratch sort_sc;
     double start_s = 0;
     std::atomic<int> finished{0};
     double done_s = 0;
@@ -202,6 +210,7 @@ struct Serve {
     std::mutex pm;  // guards `pin`; never held across a HIP call (taken after `m`)
     void* pin_mem = nullptr;
     Arena pin;      // read-back buffers in the pinned block
+    uint64_t sort_bytes = 0;  // device memory of the lanes' depth-sort buffers (worker thread only)
 };
 
 static void waiter_main(Serve* sp, Lane* l) {
@@ -269,7 +278,7 @@ static uint64_t block_bytes(const Plan& p, uint32_t extra, uint32_t runs) {
     B.extra_cap = extra;
     B.runs_cap = runs;
     const uint64_t desc = align_up(p.n_draws * sizeof(DrawDesc)) + align_up(p.n_frames * sizeof(FrameDesc));
-    return desc + scratch_bytes(B) + p.targets;
+    return desc + (p.sorted ? 0 : scratch_bytes(B)) + p.targets;
 }
 
 static Plan plan_of(const std::vector<Req>& reqs) {
@@ -354,10 +363,189 @@ static bool take_block(Serve& s, Lane& l) {
     return true;
 }
 
+// A served frame's descriptor, its targets taken from `tgt` onwards (which
+// advances past them).
+static FrameDesc frame_desc(const Req& q, uint8_t*& tgt, uint32_t& tile_base) {
+    const uint64_t px = (uint64_t)q.r.width * q.r.height;
+    FrameDesc f;
+    std::memset(&f, 0, sizeof(f));
+    f.color = tgt;
+    tgt += align_up(px * bytes_per_pixel(q.r.fmt));
+    if (q.r.with_depth) {
+        f.depth = (float*)tgt;
+        tgt += align_up(px * 4);
+    }
+    f.color_stride = (uint64_t)q.r.width * bytes_per_pixel(q.r.fmt);
+    f.depth_stride = q.r.width;
+    f.width = q.r.width;
+    f.height = q.r.height;
+    f.fmt = q.r.fmt;
+    rf_ctx c = q.r.ctx;
+    c.clear_color = 1;  // a served frame starts from Frame::clear
+    c.clear_depth = 1;
+    fill_frame_ctx(f, c);
+    set_grid(f, SERVE_TILE_W, DEFAULT_TILE_H, tile_base);
+    return f;
+}
+
+// A draw's descriptor (vert_base and tri_base left at 0).
+static DrawDesc draw_desc(const rf_mesh_draw& md, uint32_t fi) {
+    const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
+    const Tex* t = reinterpret_cast<const Tex*>(md.tex);
+    DrawDesc dd;
+    std::memset(&dd, 0, sizeof(dd));
+    dd.pos = (const float*)m->pos.p;
+    dd.attr = (const float*)m->attr.p;
+    dd.tris = (const uint32_t*)m->tris.p + (m->prim == PRIM_EDGE ? 2 : 3) * md.first_tri;
+    dd.prim = m->prim;
+    dd.tex = t ? t->texels() : nullptr;
+    dd.n_verts = (uint32_t)m->n_verts;
+    dd.n_tris = (uint32_t)(md.n_tris ? md.n_tris : m->n_tris - md.first_tri);
+    dd.kind = md.kind;
+    dd.frame = fi;
+    dd.tex_w = t ? t->w : 0;
+    dd.tex_h = t ? t->h : 0;
+    dd.tex_stride = t ? t->stride : 0;
+    dd.flat_rgba = pack_pixel(PIX_RGBA8, md.flat_color[0], md.flat_color[1], md.flat_color[2], md.flat_color[3]);
+    dd.attr_stride = m->n_attr;
+    std::memcpy(dd.mvp, md.mvp, 64);
+    std::memcpy(dd.vp, md.viewport, 64);
+    dd.cull = md.cull;
+    std::memcpy(dd.bbox, m->bbox, sizeof(dd.bbox));
+    return dd;
+}
+
+// Grows the lane's pinned descriptor staging and stats to hold `desc_bytes`
+// and `frames` (host memory; sized at open, grown only past that).
+static rf_status stage_room(Lane& l, size_t desc_bytes, uint32_t frames) {
+    if (desc_bytes > l.h_desc_cap) {
+        if (l.h_desc) (void)hipHostFree(l.h_desc);
+        l.h_desc = nullptr;
+        l.h_desc_cap = 0;
+        if (hipHostMalloc((void**)&l.h_desc, desc_bytes * 2, 0)) return RF_E_DEVICE;
+        l.h_desc_cap = desc_bytes * 2;
+    }
+    if (frames > l.stats_cap) {
+        if (l.h_stats) (void)hipHostFree(l.h_stats);
+        l.h_stats = nullptr;
+        l.stats_cap = 0;
+        if (hipHostMalloc((void**)&l.h_stats, (size_t)frames * 2 * sizeof(FrameStats), 0)) return RF_E_DEVICE;
+        l.stats_cap = frames * 2;
+    }
+    return RF_OK;
+}
+
+// A depth-sorted request (alone in its launch; render.rs:180-183, 209-219):
+// its triangle draws go through the lane's sort pass, like rf_render's, and
+// the pipeline runs on the lane's own grow-only scratch, synchronously (the
+// sort pass learns the fan counts first); the block holds the descriptors
+// and the targets. Then the read-back and the completion event, as for any
+// launch.
+static rf_status launch_lane_sorted(Serve& s, Lane& l) {
+    Device& d = *s.dev;
+    const Req& q = l.reqs[0];
+    uint8_t* blk = (uint8_t*)s.arena.base + l.blk_off;
+    const uint64_t db = align_up(q.draws.size() * sizeof(DrawDesc)), fb = align_up(sizeof(FrameDesc));
+    uint8_t* tgt = blk + db + fb;
+    uint32_t tile_base = 0;
+    const FrameDesc f = frame_desc(q, tgt, tile_base);
+    if (rf_status st = stage_room(l, db + fb, 1)) return st;
+    std::memcpy(l.h_desc + db, &f, sizeof(f));
+    l.start_s = now_s();
+    if (hipMemcpyAsync(blk + db, l.h_desc + db, sizeof(f), hipMemcpyHostToDevice, l.stream)) return RF_E_DEVICE;
+    // the sort pass over the request's triangle draws
+    SortPass& S = l.sp;
+    S.draws.clear();
+    std::vector<int32_t> sort_of;
+    for (const rf_mesh_draw& md : q.draws) {
+        const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
+        if (m->prim == PRIM_TRI) {
+            sort_of.push_back((int32_t)S.draws.size());
+            DrawDesc dd = draw_desc(md, 0);
+            dd.sort = (uint32_t)q.r.ctx.depth_sort;
+            S.draws.push_back(dd);
+        } else {
+            sort_of.push_back(-1);
+        }
+    }
+    S.frames = (const FrameDesc*)(blk + db);
+    S.frames_n = 1;
+    S.main_of.assign(S.draws.size(), 0);
+    uint32_t e = 0;
+    rf_status st = sortpass_plan(d, S, l.stream, &s.sort_bytes, &e);
+    if (!st) st = map_err(e);
+    if (st) return st;
+    // the main draws: the sorted fans of each triangle draw as its vertex
+    // slots and index triples
+    std::vector<DrawDesc> dds;
+    uint64_t vbase = 0, tbase = 0;
+    uint32_t any_cull = 0;
+    for (size_t k = 0; k < q.draws.size(); ++k) {
+        DrawDesc dd = draw_desc(q.draws[k], 0);
+        if (const int32_t si = sort_of[k]; si >= 0) {
+            const uint32_t nf = S.cnt[si];
+            dd.pre = 1;
+            dd.sort = (uint32_t)q.r.ctx.depth_sort;
+            dd.in_tris = dd.n_tris;
+            dd.in_verts = dd.n_verts;
+            dd.n_tris = nf;
+            dd.n_verts = 3 * nf;
+            dd.tris = sortpass_idx(S, S.off[si]);
+            S.main_of[si] = (uint32_t)k;
+        }
+        dd.vert_base = vbase;
+        dd.tri_base = tbase;
+        any_cull |= dd.cull;
+        vbase += dd.n_verts;
+        tbase += dd.n_tris;
+        dds.push_back(dd);
+    }
+    if (tbase >= MAX_LAUNCH_PRIMS) return RF_E_UNSUPPORTED;
+    if (sortpass_map(S, l.stream)) return RF_E_DEVICE;
+    std::memcpy(l.h_desc, dds.data(), dds.size() * sizeof(DrawDesc));
+    if (hipMemcpyAsync(blk, l.h_desc, dds.size() * sizeof(DrawDesc), hipMemcpyHostToDevice, l.stream))
+        return RF_E_DEVICE;
+    BatchDesc B;
+    std::memset(&B, 0, sizeof(B));
+    B.draws = (const DrawDesc*)blk;
+    B.frames = (const FrameDesc*)(blk + db);
+    B.n_draws = (uint32_t)dds.size();
+    B.n_frames = 1;
+    B.n_verts = vbase;
+    B.n_tris = tbase;
+    B.n_groups = (uint32_t)((tbase + WAVE_SIZE - 1) / WAVE_SIZE);
+    B.n_tiles = tile_base;
+    B.tile_w = SERVE_TILE_W;
+    B.tile_h = DEFAULT_TILE_H;
+    B.chunk = RF_SERVE_CHUNK;
+    B.sort_tpw = 1;
+    B.any_cull = any_cull;
+    B.extra_cap = l.extra_cap;
+    B.runs_cap = std::max<uint32_t>(4 * B.n_groups + B.n_tiles + 1024, l.runs_cap);
+    std::vector<FrameStats> fs;
+    uint32_t err = 0;
+    st = run_sized(d, l.sort_sc, B, l.stream,
+                   [&S, &l](const BatchDesc& M) -> hipError_t {
+                       hipError_t r = sortpass_fans(S, l.stream);
+                       return r ? r : sortpass_sort_gather(S, M, l.stream);
+                   },
+                   fs, &err, nullptr, &s.sort_bytes);
+    if (!st) st = map_err(err);
+    if (st) return st;
+    l.B = B;
+    l.h_stats[0] = fs[0];
+    for (int k = 0; k < 8; ++k) l.h_misc[k] = 0;  // errors and capacities: settled by run_sized
+    if (hipMemcpyAsync(l.pix[0], f.color, l.pix_bytes[0], hipMemcpyDeviceToHost, l.stream)) return RF_E_DEVICE;
+    l.h_misc[8] = 0;
+    if (hipEventRecord(l.ev, l.stream)) return RF_E_DEVICE;
+    return RF_OK;
+}
+
 // Lays the lane's block out (descriptors, scratch, targets), uploads the
 // descriptors and enqueues the pipeline, the readbacks and the completion
 // event. Called by the worker without s.m.
 static rf_status launch_lane(Serve& s, Lane& l) {
+    if (l.reqs.size() == 1 && l.reqs[0].sorted) return launch_lane_sorted(s, l);
     Device& d = *s.dev;
     uint8_t* blk = (uint8_t*)s.arena.base + l.blk_off;
     const Plan p = plan_of(l.reqs);
@@ -390,77 +578,20 @@ static rf_status launch_lane(Serve& s, Lane& l) {
     uint32_t tile_base = 0, any_cull = 0;
     for (size_t i = 0; i < l.reqs.size(); ++i) {
         Req& q = l.reqs[i];
-        const uint64_t px = (uint64_t)q.r.width * q.r.height;
-        FrameDesc f;
-        std::memset(&f, 0, sizeof(f));
-        f.color = tgt;
-        tgt += align_up(px * bytes_per_pixel(q.r.fmt));
-        if (q.r.with_depth) {
-            f.depth = (float*)tgt;
-            tgt += align_up(px * 4);
-        }
-        f.color_stride = (uint64_t)q.r.width * bytes_per_pixel(q.r.fmt);
-        f.depth_stride = q.r.width;
-        f.width = q.r.width;
-        f.height = q.r.height;
-        f.fmt = q.r.fmt;
-        rf_ctx c = q.r.ctx;
-        c.clear_color = 1;  // a served frame starts from Frame::clear
-        c.clear_depth = 1;
-        fill_frame_ctx(f, c);
-        set_grid(f, SERVE_TILE_W, DEFAULT_TILE_H, tile_base);
-        fds.push_back(f);
+        fds.push_back(frame_desc(q, tgt, tile_base));
         for (const rf_mesh_draw& md : q.draws) {
-            const Mesh* m = reinterpret_cast<const Mesh*>(md.mesh);
-            const Tex* t = reinterpret_cast<const Tex*>(md.tex);
-            DrawDesc dd;
-            std::memset(&dd, 0, sizeof(dd));
-            const uint64_t nt = md.n_tris ? md.n_tris : m->n_tris - md.first_tri;
-            dd.pos = (const float*)m->pos.p;
-            dd.attr = (const float*)m->attr.p;
-            dd.tris = (const uint32_t*)m->tris.p + (m->prim == PRIM_EDGE ? 2 : 3) * md.first_tri;
-            dd.prim = m->prim;
-            dd.tex = t ? t->texels() : nullptr;
-            dd.n_verts = (uint32_t)m->n_verts;
-            dd.n_tris = (uint32_t)nt;
-            dd.kind = md.kind;
-            dd.frame = (uint32_t)i;
-            dd.tex_w = t ? t->w : 0;
-            dd.tex_h = t ? t->h : 0;
-            dd.tex_stride = t ? t->stride : 0;
-            dd.flat_rgba = pack_pixel(PIX_RGBA8, md.flat_color[0], md.flat_color[1], md.flat_color[2],
-                                      md.flat_color[3]);
-            dd.attr_stride = m->n_attr;
+            DrawDesc dd = draw_desc(md, (uint32_t)i);
             dd.vert_base = vbase;
             dd.tri_base = tbase;
-            std::memcpy(dd.mvp, md.mvp, 64);
-            std::memcpy(dd.vp, md.viewport, 64);
-            dd.cull = md.cull;
-            std::memcpy(dd.bbox, m->bbox, sizeof(dd.bbox));
             any_cull |= md.cull;
-            vbase += m->n_verts;
-            tbase += nt;
+            vbase += dd.n_verts;
+            tbase += dd.n_tris;
             dds.push_back(dd);
         }
     }
     B.any_cull = any_cull;
-    // pinned staging of the descriptors, sized at open; it grows only for a
-    // launch of more draws than that (host memory, never the device's)
     const size_t total = db + fb;
-    if (total > l.h_desc_cap) {
-        if (l.h_desc) (void)hipHostFree(l.h_desc);
-        l.h_desc = nullptr;
-        l.h_desc_cap = 0;
-        if (hipHostMalloc((void**)&l.h_desc, total * 2, 0)) return RF_E_DEVICE;
-        l.h_desc_cap = total * 2;
-    }
-    if ((uint32_t)fds.size() > l.stats_cap) {
-        if (l.h_stats) (void)hipHostFree(l.h_stats);
-        l.h_stats = nullptr;
-        l.stats_cap = 0;
-        if (hipHostMalloc((void**)&l.h_stats, fds.size() * 2 * sizeof(FrameStats), 0)) return RF_E_DEVICE;
-        l.stats_cap = (uint32_t)fds.size() * 2;
-    }
+    if (rf_status st = stage_room(l, total, (uint32_t)fds.size())) return st;
     std::memcpy(l.h_desc, dds.data(), dds.size() * sizeof(DrawDesc));
     std::memcpy(l.h_desc + db, fds.data(), fds.size() * sizeof(FrameDesc));
     l.B = B;
@@ -570,8 +701,8 @@ static void worker_main(Serve* sp) {
                 }
                 const double c = (double)q.r.width * q.r.height + COST_TRI * (double)q.n_tris;
                 const double cheap = take.empty() ? c : std::min(cheapest, c);
-                if (!take.empty() && cost + c > SERVE_SLOWDOWN * (COST_LAUNCH + cheap)) {
-                    ++i;  // another cost class: a later launch
+                if (!take.empty() && (q.sorted || cost + c > SERVE_SLOWDOWN * (COST_LAUNCH + cheap))) {
+                    ++i;  // another cost class, or depth-sorted: a later launch
                     continue;
                 }
                 Plan p2 = p;
@@ -590,6 +721,7 @@ static void worker_main(Serve* sp) {
                 cheapest = cheap;
                 take.push_back(i);
                 ++i;
+                if (q.sorted) break;  // a depth-sorted request goes alone
             }
             for (size_t k : take) l->reqs.push_back(std::move(s.pending[k]));
             for (auto it = take.rbegin(); it != take.rend(); ++it) s.pending.erase(s.pending.begin() + (ptrdiff_t)*it);
@@ -708,7 +840,6 @@ rf_status rf_serve_submit(rf_serve* sh, const rf_request* req) {
         return RF_E_ARG;
     Serve& s = *reinterpret_cast<Serve*>(sh);
     rf_status st = check_ctx(&req->ctx);
-    if (!st && req->ctx.depth_sort) st = RF_E_UNSUPPORTED;  // depth sort: rf_render only
     if (st) return st;
     Req q;
     q.r = *req;
@@ -724,6 +855,7 @@ rf_status rf_serve_submit(rf_serve* sh, const rf_request* req) {
             return st;
         q.n_tris += md.n_tris ? md.n_tris : m->n_tris - md.first_tri;
         q.n_verts += m->n_verts;
+        q.sorted = q.sorted || (req->ctx.depth_sort && m->prim == PRIM_TRI);
     }
     if (q.n_tris >= MAX_LAUNCH_PRIMS) return RF_E_UNSUPPORTED;
     q.submit_s = now_s();
@@ -781,6 +913,8 @@ rf_status rf_serve_close(rf_serve* sh) {
     (void)hipSetDevice(d.ordinal);
     for (Lane* l : s->lanes) {
         if (l->stream) (void)hipStreamSynchronize(l->stream);
+        sortpass_release(l->sp, &s->sort_bytes);
+        l->sort_sc.release(&s->sort_bytes);
         if (l->h_desc) (void)hipHostFree(l->h_desc);
         if (l->h_stats) (void)hipHostFree(l->h_stats);
         if (l->h_misc) (void)hipHostFree(l->h_misc);

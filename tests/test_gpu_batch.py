@@ -415,5 +415,77 @@ def test_depth_sorted_frames_in_batches(sort, dt):
     dtex.free()
 
 
+@pytest.mark.parametrize("sort", ["front_to_back", "back_to_front"])
+@pytest.mark.parametrize("dt", [None, "less"])
+def test_depth_sorted_served_frames(sort, dt):
+    """Served requests whose context asks for a depth sort (render.rs:180-183,
+    209-219), beside requests without one: each sorted request is launched
+    alone through its stream's sort pass; every frame (colour and stats)
+    equals the oracle."""
+    import time
+    from test_gpu_parity import random_soup
+    rng = np.random.default_rng(170 + (sort == "back_to_front") + 2 * (dt is None))
+    tex = rng.integers(0, 256, size=(32, 64, 4), dtype=np.uint8)
+    dtex = RB.Texture(tex)
+    shaders = {0: rf.shader.Flat((9, 200, 77, 255)), 1: rf.shader.Gouraud(), 2: rf.shader.TexClamp(tex),
+               3: rf.shader.TexRepeatPot(tex)}
+    meshes, reqs, oracle = [], [], []
+    for fi in range(8):
+        w, h = [(301, 197), (256, 256), (128, 96)][fi % 3]
+        sorted_frame = fi % 4 != 2
+        ctx = rf.Context(clear=True, depth_test=dt, face_cull=None, depth_sort=sort if sorted_frame else None)
+        ocol = np.zeros((h, w, 4), np.uint8)
+        ocol[...] = (0, 0, 0, 255)
+        odep = np.zeros((h, w), np.float32)
+        ost = np.zeros(8, np.uint64)
+        draws = []
+        for k, kind in enumerate([1, 3, 0, 2]):
+            pos, attr, tris, mvp, vp = random_soup(rng, 60 + 30 * k, w, h, kind)
+            mesh = RB.Mesh(pos, tris, attr)
+            meshes.append(mesh)
+            draws.append(RB.mesh_draw(mesh, shaders[kind], mvp, vp, texture=dtex if kind >= 2 else None))
+            e, _ = O.render(kind, tris, pos, attr, mvp, vp, ocol, depth=odep, flat=(9, 200, 77, 255),
+                            tex=tex if kind >= 2 else None,
+                            ctx=(DT_[dt], 0, 1, 1, DS_[sort] if sorted_frame else 0), stats=ost)
+            assert e == 0
+        # an edge draw keeps its order under the sort
+        epos = rng.uniform(-1, 1, size=(80, 3)).astype(np.float32)
+        ecol = rng.uniform(0, 1, size=(80, 3)).astype(np.float32)
+        edges = np.arange(80, dtype=np.uint32).reshape(40, 2)
+        emesh = RB.Mesh(epos, edges, ecol)
+        meshes.append(emesh)
+        draws.append(RB.mesh_draw(emesh, rf.shader.Gouraud(), M.identity(), M.viewport(0, h, w, 0)))
+        e, _ = O.render(1, edges, epos, ecol, M.identity(), M.viewport(0, h, w, 0), ocol, depth=odep,
+                        ctx=(DT_[dt], 0, 1, 1, DS_[sort] if sorted_frame else 0), stats=ost, prim=1)
+        assert e == 0
+        reqs.append((w, h, ctx, draws))
+        oracle.append((ocol, ost))
+    srv = RB.Server(budget_bytes=256 << 20, max_batch=4, n_streams=2)
+    got = {}
+    try:
+        for k, (w, h, ctx, draws) in enumerate(reqs):
+            srv.submit(k, draws, w, h, ctx=ctx)
+        t_end = time.monotonic() + 60.0
+        while len(got) < len(reqs):
+            assert time.monotonic() < t_end, f"results missing: {sorted(set(range(len(reqs))) - set(got))}"
+            bufs = [np.empty(301 * 256 * 4, np.uint8) for _ in range(4)]
+            for r, b in zip(srv.poll(4, timeout_ms=500, buffers=bufs), bufs):
+                assert r.status == 0, f"request {r.id} status {r.status}"
+                w, h = reqs[r.id][0], reqs[r.id][1]
+                got[r.id] = (r, b[: w * h * 4].reshape(h, w, 4).copy())
+    finally:
+        srv.close()
+    for k in range(len(reqs)):
+        r, col = got[k]
+        ocol, ost = oracle[k]
+        assert np.array_equal(col, ocol), f"request {k} colour"
+        s = r.stats
+        assert [s.calls, s.prims_i, s.prims_o, s.verts_i, s.verts_o, s.frags_i, s.frags_o] == \
+            [int(x) for x in ost[:7]], f"request {k} stats"
+    for m in meshes:
+        m.free()
+    dtex.free()
+
+
 DT_ = {None: 0, "less": 1, "equal": 2, "greater": 3}
 DS_ = {None: 0, "front_to_back": 1, "back_to_front": 2}

@@ -285,19 +285,6 @@ def test_depth_sort_crates_and_edges():
     assert_same(o, g)
 
 
-def test_depth_sort_rejected_by_the_serving_loop():
-    m = RB.Mesh(np.zeros((3, 3), np.float32), np.array([[0, 1, 2]], np.uint32))
-    srv = RB.Server(budget_bytes=1 << 24, max_batch=2, n_streams=1)
-    try:
-        with pytest.raises(rf.RenderError) as e:
-            srv.submit(1, [RB.mesh_draw(m, rf.shader.Flat(), M.identity(), M.viewport(0, 16, 16, 0))], 16, 16,
-                       ctx=rf.Context(depth_sort="front_to_back"))
-        assert e.value.status == -5
-    finally:
-        srv.close()
-    m.free()
-
-
 @pytest.mark.parametrize("fmt", [rf.PIX_RGB8, rf.PIX_U32_ARGB, rf.PIX_RGB565])
 def test_depth_sort_pixel_formats(fmt):
     rng = np.random.default_rng(77)
