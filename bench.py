@@ -38,6 +38,7 @@ def parse():
     p.add_argument("--tile-w", type=int, default=0, help="0 = autotune")
     p.add_argument("--serve-seconds", type=float, default=2.0)
     p.add_argument("--serve-rate", type=float, default=1000.0)
+    p.add_argument("--serve-streams", type=int, default=6, help="concurrent launches of the serving loop")
     p.add_argument("--serve-sweep-seconds", type=float, default=5.0,
                    help="seconds per offered rate of the 16-mesh serving sweep (0 = no sweep)")
     p.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -230,7 +231,7 @@ def crates_request(RB, mesh, shd, texr, w, h, yaw):
     return [RB.mesh_draw(mesh, shd, S.crates_mvp(w, h, yaw), M.viewport(0, h, w, 0), texture=texr)]
 
 
-def serve_stream(RB, reqs, seconds, warm, budget=4 << 30, alone=None):
+def serve_stream(RB, reqs, seconds, warm, budget=4 << 30, alone=None, streams=6):
     """Open-loop replay of (time, w, h, draws) requests into a fresh server;
     latency = completion - submission, wait = admission - submission. `warm`
     (w, h, draws) requests run first, untimed, so the pinned readback pool
@@ -238,7 +239,7 @@ def serve_stream(RB, reqs, seconds, warm, budget=4 << 30, alone=None):
     given, holds each request's latency when served on its own (ms): the
     slowdown latency / alone is then reported too."""
     clk = {"open_ns": time.clock_gettime_ns(time.CLOCK_MONOTONIC)}
-    srv = RB.Server(budget_bytes=budget, max_batch=32, n_streams=3)
+    srv = RB.Server(budget_bytes=budget, max_batch=32, n_streams=streams)
     n = len(reqs)
     for k, (w, h, draws) in enumerate(warm):
         srv.submit(10 ** 9 + k, draws, w, h)
@@ -333,7 +334,8 @@ def serve_run(rf, RB, mesh, texr, shd, a):
         t += -math.log(max(1e-7, 1.0 - float(rng.uniform_f32(0.0, 1.0)))) / a.serve_rate
         w, h = S.RESOLUTIONS[rng.uniform_usize(0, 5)]
         reqs.append((t, w, h, crates_request(RB, mesh, shd, texr, w, h, float(rng.uniform_f32(0.0, 1.0)))))
-    out = serve_stream(RB, reqs, a.serve_seconds, [(w, h, reqs[0][3]) for (w, h) in S.RESOLUTIONS])
+    out = serve_stream(RB, reqs, a.serve_seconds, [(w, h, reqs[0][3]) for (w, h) in S.RESOLUTIONS],
+                       streams=a.serve_streams)
     out["offered_rate"] = a.serve_rate
     out["resolutions"] = "uniform over 256^2, 512^2, 1280x720, 1920x1080, 2048^2"
     if a.serve_sweep_seconds <= 0:
@@ -357,7 +359,7 @@ def serve_run(rf, RB, mesh, texr, shd, a):
         spec = S.serve_stream(rate, max(1, int(rate * a.serve_sweep_seconds)), seed=5)
         reqs = [(t_, w, h, draws_of(w, h, mi, pose)) for (t_, w, h, mi, pose) in spec]
         r = serve_stream(RB, reqs, a.serve_sweep_seconds, warm,
-                         alone=[alone_cls[(w, h, mi)] for (_, w, h, mi, _) in spec])
+                         alone=[alone_cls[(w, h, mi)] for (_, w, h, mi, _) in spec], streams=a.serve_streams)
         r["offered_rate"] = rate
         sweep.append(r)
     # the plain sequential GPU call on the same kind of requests
