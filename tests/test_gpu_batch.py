@@ -487,5 +487,73 @@ def test_depth_sorted_served_frames(sort, dt):
     dtex.free()
 
 
+_FMT_ARRAYS = {rf.PIX_RGBA8: (np.uint8, 4), rf.PIX_RGB8: (np.uint8, 3), rf.PIX_U32_RGBA: (np.uint32, 0),
+               rf.PIX_U32_XRGB: (np.uint32, 0), rf.PIX_U32_ARGB: (np.uint32, 0), rf.PIX_U32_BGRA: (np.uint32, 0),
+               rf.PIX_RGBA4444: (np.uint16, 0), rf.PIX_RGB565: (np.uint16, 0)}
+
+
+@pytest.mark.parametrize("fmt", sorted(_FMT_ARRAYS))
+def test_pixel_formats_batched_and_served(world, fmt):
+    """Every pixel format through the batched and the serving path, with and
+    without depth, two draws per frame (textured crates, then Gouraud
+    icospheres): each frame equals the plain path, whose formats are checked
+    against the oracle."""
+    import time
+    crates = RB.Mesh(world["cpos"], world["ctris"], world["cuv"])
+    icos = RB.Mesh(world["ipos"], world["itris"], world["iattr"])
+    tex = RB.Texture(world["tex"])
+    shd = rf.shader.TexRepeatPot(world["tex"])
+    dt, ch = _FMT_ARRAYS[fmt]
+    frames = []
+    for k, (w, h, with_depth) in enumerate([(320, 200, True), (256, 160, False)]):
+        mvp_c = M.then(S.first_person(0.0, 1.7, 0.0, 0.15 * k), _persp(w, h))
+        view_i = M.then(M.rotate_y(0.3 * k), M.translate3(0.0, 0.0, 2.5))
+        mvp_i = M.then(view_i, M.perspective(np.float32(1.7320508), np.float32(w) / np.float32(h), 0.1, 100.0))
+        vp = M.viewport(0, h, w, 0)
+        col = np.zeros((h, w, ch) if ch else (h, w), dt)
+        dep = np.zeros((h, w), np.float32) if with_depth else None
+        tgt = rf.Framebuf(col, dep, fmt) if with_depth else rf.Colorbuf(col, fmt)
+        ctx = rf.Context(clear=True)
+        rf.render(world["ctris"], (world["cpos"], world["cuv"]), shd, mvp_c, vp, tgt, ctx)
+        ctx.clear = False
+        rf.render(world["itris"], (world["ipos"], world["iattr"]), rf.shader.Gouraud(), mvp_i, vp, tgt, ctx)
+        draws = [RB.mesh_draw(crates, shd, mvp_c, vp, texture=tex), RB.mesh_draw(icos, rf.shader.Gouraud(), mvp_i, vp)]
+        frames.append((w, h, with_depth, draws, col, dep))
+    # batched
+    fbs = [RB.DeviceFramebuf(w, h, fmt, with_depth) for (w, h, with_depth, _, _, _) in frames]
+    b = RB.Batch([(fb, rf.Context(clear=True), f[3]) for fb, f in zip(fbs, frames)])
+    b.run(sync=True)
+    b.wait()
+    for k, (fb, f) in enumerate(zip(fbs, frames)):
+        gcol, gdep = fb.read()
+        assert np.array_equal(gcol, f[4]), f"batched frame {k} colour"
+        if f[2]:
+            assert np.array_equal(gdep.view(np.uint32), f[5].view(np.uint32)), f"batched frame {k} depth"
+    b.free()
+    for fb in fbs:
+        fb.free()
+    # served
+    srv = RB.Server(budget_bytes=256 << 20, max_batch=4, n_streams=2)
+    got = {}
+    try:
+        for k, (w, h, with_depth, draws, _, _) in enumerate(frames):
+            srv.submit(k, draws, w, h, fmt=fmt, with_depth=with_depth)
+        t_end = time.monotonic() + 60.0
+        while len(got) < len(frames):
+            assert time.monotonic() < t_end, "results missing"
+            bufs = [np.empty(320 * 200 * 4, np.uint8) for _ in range(2)]
+            for r, buf in zip(srv.poll(2, timeout_ms=500, buffers=bufs), bufs):
+                assert r.status == 0, f"request {r.id} status {r.status}"
+                got[r.id] = buf
+    finally:
+        srv.close()
+    for k, f in enumerate(frames):
+        want = f[4].tobytes()
+        assert got[k][: len(want)].tobytes() == want, f"served frame {k} colour"
+    crates.free()
+    icos.free()
+    tex.free()
+
+
 DT_ = {None: 0, "less": 1, "equal": 2, "greater": 3}
 DS_ = {None: 0, "front_to_back": 1, "back_to_front": 2}
