@@ -555,5 +555,72 @@ def test_pixel_formats_batched_and_served(world, fmt):
     tex.free()
 
 
+_CTX_VARIANTS = [
+    dict(depth_test="less", face_cull="back", color_clear=(10, 20, 30, 255), depth_clear=50.0),
+    dict(depth_test=None, face_cull=None),
+    dict(depth_test="greater", face_cull="front", depth_clear=0.5),
+    dict(depth_test="equal", face_cull=None),
+    dict(depth_test="less", face_cull=None, color_write=False, color_clear=(1, 2, 3, 4)),
+    dict(depth_test="less", face_cull=None, depth_write=False),
+]
+
+
+def test_contexts_batched_and_served(world):
+    """Each Context setting (depth tests, face culling, colour or depth writes
+    off, clear values) on its own frame of one batch and of one served
+    stream: every frame equals the plain path."""
+    import time
+    crates = RB.Mesh(world["cpos"], world["ctris"], world["cuv"])
+    icos = RB.Mesh(world["ipos"], world["itris"], world["iattr"])
+    tex = RB.Texture(world["tex"])
+    shd = rf.shader.TexRepeatPot(world["tex"])
+    w, h = 256, 160
+    vp = M.viewport(0, h, w, 0)
+    frames = []
+    for k, cv in enumerate(_CTX_VARIANTS):
+        mvp_c = M.then(S.first_person(0.0, 1.7, 0.0, 0.1 * k), _persp(w, h))
+        view_i = M.then(M.rotate_y(0.2 * k), M.translate3(0.0, 0.0, 2.5))
+        mvp_i = M.then(view_i, M.perspective(np.float32(1.7320508), np.float32(w) / np.float32(h), 0.1, 100.0))
+        col = np.zeros((h, w, 4), np.uint8)
+        dep = np.zeros((h, w), np.float32)
+        ctx = rf.Context(clear=True, **cv)
+        rf.render(world["ctris"], (world["cpos"], world["cuv"]), shd, mvp_c, vp, rf.Framebuf(col, dep), ctx)
+        ctx.clear = False
+        rf.render(world["itris"], (world["ipos"], world["iattr"]), rf.shader.Gouraud(), mvp_i, vp,
+                  rf.Framebuf(col, dep), ctx)
+        draws = [RB.mesh_draw(crates, shd, mvp_c, vp, texture=tex), RB.mesh_draw(icos, rf.shader.Gouraud(), mvp_i, vp)]
+        frames.append((rf.Context(clear=True, **cv), draws, col, dep))
+    fbs = [RB.DeviceFramebuf(w, h, 0, True) for _ in frames]
+    b = RB.Batch([(fb, f[0], f[1]) for fb, f in zip(fbs, frames)])
+    b.run(sync=True)
+    b.wait()
+    for k, (fb, f) in enumerate(zip(fbs, frames)):
+        gcol, gdep = fb.read()
+        assert np.array_equal(gcol, f[2]), f"batched frame {k} ({_CTX_VARIANTS[k]}) colour"
+        assert np.array_equal(gdep.view(np.uint32), f[3].view(np.uint32)), f"batched frame {k} depth"
+    b.free()
+    for fb in fbs:
+        fb.free()
+    srv = RB.Server(budget_bytes=256 << 20, max_batch=4, n_streams=2)
+    got = {}
+    try:
+        for k, (ctx, draws, _, _) in enumerate(frames):
+            srv.submit(k, draws, w, h, ctx=ctx)
+        t_end = time.monotonic() + 60.0
+        while len(got) < len(frames):
+            assert time.monotonic() < t_end, "results missing"
+            bufs = [np.empty(w * h * 4, np.uint8) for _ in range(3)]
+            for r, buf in zip(srv.poll(3, timeout_ms=500, buffers=bufs), bufs):
+                assert r.status == 0, f"request {r.id} status {r.status}"
+                got[r.id] = buf.reshape(h, w, 4)
+    finally:
+        srv.close()
+    for k, f in enumerate(frames):
+        assert np.array_equal(got[k], f[2]), f"served frame {k} ({_CTX_VARIANTS[k]}) colour"
+    crates.free()
+    icos.free()
+    tex.free()
+
+
 DT_ = {None: 0, "less": 1, "equal": 2, "greater": 3}
 DS_ = {None: 0, "front_to_back": 1, "back_to_front": 2}
